@@ -1,0 +1,203 @@
+/*
+ * ccfv.h — C ABI of the MI355X (gfx950) T2S4 CC-CFV solver and its Parareal driver.
+ *
+ * Everything here is plain C: doubles, fixed-width integers, sizes and opaque
+ * handles.  No HIP or torch types cross the boundary.  All state is FP64.
+ *
+ * The reference program (T2S4_Parareal.cu) has no library surface: its solver
+ * lives behind four internal C++ entry points, listed next to their
+ * replacements below:
+ *
+ *   GpuOneStep(GPUContext&, FLOAT tn, FLOAT dt)          T2S4_Parareal.cu:785   -> ccfv_step
+ *   CoarsePropagator(GPUContext&, FLOAT tn, FLOAT DT)    T2S4_Parareal.cu:971   -> ccfv_coarse
+ *   FinePropagator(GPUContext&, FLOAT tn, FLOAT DT, J)   T2S4_Parareal.cu:977   -> ccfv_fine
+ *   parareal_correction_kernel(Gk1, Gk, F, U, N)         T2S4_Parareal.cu:771   -> ccfv_parareal_correction
+ *
+ * plus the two hot loops of main():
+ *
+ *   serial fine loop                                     T2S4_Parareal.cu:1234  -> ccfv_plain_solve
+ *   Parareal init + iterations                           T2S4_Parareal.cu:1250  -> ccfv_parareal_solve
+ *
+ * and the context set-up of main() (config keys N, K, T_min, T_max, Nt at
+ * T2S4_Parareal.cu:1031-1049, buffers at :1088-1135, GPUContext at :724).
+ *
+ * State layout in memory is the reference's: C[i*ny + j] with i the x index
+ * (slow) and j the y index (contiguous), T2S4_Parareal.cu:176-180.
+ *
+ * Every function returns an int status (CCFV_OK == 0, negative on error) and
+ * never aborts; ccfv_last_error() returns a thread-local message describing
+ * the most recent failure on the calling thread.
+ */
+#ifndef CCFV_H
+#define CCFV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define CCFV_OK          0
+#define CCFV_EINVAL     (-1)   /* bad argument or shape                         */
+#define CCFV_ENOMEM     (-2)   /* device or host allocation failed              */
+#define CCFV_EDEVICE    (-3)   /* HIP runtime error / no device                 */
+#define CCFV_ENOTCONV   (-4)   /* Parareal stopped at kmax above tolerance      */
+#define CCFV_ECFL       (-5)   /* characteristic feet crossed (time step too large) */
+#define CCFV_EBUSY      (-6)   /* serving: request refused by the memory budget */
+#define CCFV_ENONFINITE (-7)   /* NaN/Inf detected in a state                   */
+
+/* ---- boundary mode ----------------------------------------------------- */
+/* CCFV_BC_REFERENCE: the reference's closed box.  Feet are clipped to the box
+ * and cells whose feet leave it get zero inflow (T2S4_Parareal.cu:322-324),
+ * ghost cells use the zero-slope cubic closure (:275-286), the compact RHS
+ * uses zero ghosts (:624-631) and the tridiagonal system is truncated
+ * (:666-683).
+ * CCFV_BC_PERIODIC: feet wrap, every stencil wraps, the compact RHS and the
+ * tridiagonal system are cyclic (solved by Sherman-Morrison).  The reference
+ * has no periodic mode; this is the same scheme on a torus. */
+#define CCFV_BC_REFERENCE 0
+#define CCFV_BC_PERIODIC  1
+
+/* ---- velocity fields ---------------------------------------------------- */
+/* SOLID_BODY: u = -omega*(y-cy), v = omega*(x-cx); vel = {omega, cx, cy}.
+ *             omega=4, c=(0,0) is the reference's Vx=-4y, Vy=4x (:44-45).
+ * CONSTANT:   u = vel[0], v = vel[1].
+ * SWIRL:      u =  sin^2(pi x) sin(2 pi y) cos(pi t / T),
+ *             v = -sin^2(pi y) sin(2 pi x) cos(pi t / T);   vel = {T}.      */
+#define CCFV_VEL_SOLID_BODY 0
+#define CCFV_VEL_CONSTANT   1
+#define CCFV_VEL_SWIRL      2
+
+/* ---- scheme flags -------------------------------------------------------- */
+/* Reproduce the third sub-step's explicit diffusion derivative evaluated on
+ * the remapped field (T2S4_Parareal.cu:929-932) instead of the pre-remap
+ * field used by the first two sub-steps (:834-837, :882-885). */
+#define CCFV_FLAG_STEP3_POSTREMAP  0x1u
+/* Reproduce the reference's ghost selection in the foot-derivative stencil
+ * (iL==1 takes Cm2, iL==Nx-2 takes CNp1; :476-479, :530-533). */
+#define CCFV_FLAG_DERIV_GHOST_QUIRK 0x2u
+/* Both reference-mode quirks. */
+#define CCFV_FLAGS_REFERENCE (CCFV_FLAG_STEP3_POSTREMAP | CCFV_FLAG_DERIV_GHOST_QUIRK)
+
+typedef struct ccfv_problem {
+    int32_t  nx, ny;        /* cells in x and y (reference: N, with nx == ny)  */
+    int32_t  bc;            /* CCFV_BC_*                                        */
+    int32_t  vel_kind;      /* CCFV_VEL_*                                       */
+    double   x_lo, x_hi;    /* box (reference: X_min, X_max = -1, 1; :30-31)   */
+    double   y_lo, y_hi;    /* box (reference: Y_min, Y_max = -1, 1; :32-33)   */
+    double   nu;            /* diffusion coefficient (reference config key K)  */
+    double   vel[4];        /* velocity parameters, see CCFV_VEL_*             */
+    uint32_t flags;         /* CCFV_FLAG_*                                      */
+    uint32_t reserved;
+} ccfv_problem;
+
+/* ---- initial / exact data ----------------------------------------------- */
+/* ROT_GAUSSIAN: the reference's C_Exact (:51-56) generalised to a centre of
+ *   rotation: s/(s+4 nu t) exp(-|x* - x0|^2/(s + 4 nu t)) with x* the point
+ *   rotated back by omega*t about (cx, cy).  p = {x0, y0, s, omega, cx, cy}.
+ *   Reference IC: {-0.35, 0, 0.005, 4, 0, 0}.  Cell averages by the 3x3
+ *   tensor Simpson rule of :131-136.
+ * GAUSSIANS: sum of n <= 3 Gaussians a exp(-r^2/(2 w^2)) translated by a
+ *   constant velocity (u, v) and diffused (exact on the plane):
+ *   p = {u, v, nu, (x0, y0, w, a) * n}.  Simpson cell averages.
+ * SINSIN: sin(2 pi (x - u t)) sin(2 pi (y - v t)) exp(-8 pi^2 nu t), exact
+ *   cell averages (closed form).  p = {u, v, nu}.
+ * COSINE_BELL: 0.5 (1 + cos(pi r / R)) for r < R, else 0; p = {x0, y0, R}.
+ *   Simpson cell averages (t is ignored).                                    */
+#define CCFV_IC_ROT_GAUSSIAN 0
+#define CCFV_IC_GAUSSIANS    1
+#define CCFV_IC_SINSIN       2
+#define CCFV_IC_COSINE_BELL  3
+
+typedef struct ccfv_ic {
+    int32_t kind;
+    int32_t n;              /* number of Gaussians for CCFV_IC_GAUSSIANS */
+    double  p[16];
+} ccfv_ic;
+
+/* ---- Parareal report ---------------------------------------------------- */
+#define CCFV_MAX_ITERS 64
+typedef struct ccfv_parareal_report {
+    int32_t iterations;          /* iterations performed (k)                   */
+    int32_t converged;           /* 1 if the increment fell to <= tol          */
+    int32_t n_slices, j_fine;    /* slices and fine steps per slice            */
+    double  increment[CCFV_MAX_ITERS]; /* rel. L2 increment per iteration over
+                                        * all slice end-points                */
+    double  fine_steps_run;      /* single-field fine steps executed           */
+    double  coarse_steps_run;    /* single-field coarse steps executed         */
+    double  wall_ms;             /* hipEvent time of the whole solve           */
+} ccfv_parareal_report;
+
+typedef struct ccfv_ctx ccfv_ctx;
+
+/* Thread-local text of the last error on this thread ("" if none). */
+const char *ccfv_last_error(void);
+
+/* Library and device information. */
+int ccfv_version(int32_t *major, int32_t *minor);
+int ccfv_device_count(int32_t *count);
+
+/* Fill *p with the reference problem: box [-1,1]^2, Vx=-4y, Vy=4x, closed-box
+ * boundaries, both quirks on, diffusion nu = K (T2S4_Parareal.cu:27-61,
+ * :1037-1048).  n = config key N. */
+int ccfv_problem_reference(ccfv_problem *p, int32_t n, double K);
+/* Fill *ic with the reference initial condition (C_Exact at t=0, :1147). */
+int ccfv_ic_reference(ccfv_ic *ic, double K);
+
+/* Context: owns device memory for one state and the scratch the step needs
+ * (replaces the buffers of :1088-1135 and GPUContext of :724-768).  The
+ * device is the HIP device current on the calling thread. */
+int  ccfv_create(const ccfv_problem *p, ccfv_ctx **out);
+void ccfv_destroy(ccfv_ctx *ctx);
+
+/* Copy the state in / out (host arrays of nx*ny doubles, reference layout). */
+int ccfv_set_state(ccfv_ctx *ctx, const double *host);
+int ccfv_get_state(ccfv_ctx *ctx, double *host);
+/* Device-side copies for callers that keep fields in HBM (pointers to
+ * nx*ny doubles on the context's device). */
+int ccfv_set_state_device(ccfv_ctx *ctx, const double *dev);
+int ccfv_get_state_device(ccfv_ctx *ctx, double *dev);
+/* Cell averages of an initial/exact solution at time t into the state
+ * (replaces get_exact_solution_on_gpu, :118-140).  Computed on the device. */
+int ccfv_fill_exact(ccfv_ctx *ctx, const ccfv_ic *ic, double t);
+/* Cell averages into a host array, computed on the device (state untouched). */
+int ccfv_exact_to_host(ccfv_ctx *ctx, const ccfv_ic *ic, double t, double *host);
+
+/* One Strang-split step x(h/2) -> y(h) -> x(h/2) in place (GpuOneStep, :785). */
+int ccfv_step(ccfv_ctx *ctx, double tn, double h);
+/* Coarse propagator: one step of DT (CoarsePropagator, :971). */
+int ccfv_coarse(ccfv_ctx *ctx, double tn, double DT);
+/* Fine propagator: J steps of DT/J (FinePropagator, :977). */
+int ccfv_fine(ccfv_ctx *ctx, double tn, double DT, int32_t J);
+/* Plain serial-in-time solve: nt steps of (t1-t0)/nt from t0 (:1224-1243). */
+int ccfv_plain_solve(ccfv_ctx *ctx, double t0, double t1, int32_t nt);
+
+/* Parareal over [t0, t1]: n_slices coarse steps of DT=(t1-t0)/n_slices, fine
+ * propagator J = nt_fine/n_slices steps of DT/J; iterations
+ *   U_{n+1}^{k+1} = G(U_n^{k+1}) + (F(U_n^k) - G(U_n^k))
+ * (:1250-1355).  All fine propagators of one iteration run as one batched
+ * launch per sweep; the correction is fused into the coarse step.  Stops after
+ * kmax iterations or once the relative L2 increment over all slice end-points
+ * is <= tol (tol <= 0: run all kmax iterations, as the reference does).
+ * The initial state is the context state; on return the state holds U^k(t1).
+ * Returns CCFV_OK, or CCFV_ENOTCONV when tol > 0 was not reached (the state
+ * still holds the last iterate).  rep may be NULL. */
+int ccfv_parareal_solve(ccfv_ctx *ctx, double t0, double t1, int32_t nt_fine,
+                        int32_t n_slices, int32_t kmax, double tol,
+                        ccfv_parareal_report *rep);
+
+/* U = G_new + (F - G_old) over n doubles, device pointers, on the context's
+ * stream (parareal_correction_kernel, :771-782). */
+int ccfv_parareal_correction(ccfv_ctx *ctx, const double *g_new,
+                             const double *g_old, const double *f,
+                             double *u, size_t n);
+
+/* Block until all work queued on the context's stream is done. */
+int ccfv_sync(ccfv_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CCFV_H */

@@ -1,0 +1,201 @@
+"""Python binding of the gfx950 T2S4 CC-CFV solver (C ABI: include/ccfv.h).
+
+The binding is a thin ctypes layer over lib/libccfv.so.  It mirrors the
+reference's solver entry points:
+
+  GpuOneStep(ctx, tn, dt)             T2S4_Parareal.cu:785  -> Context.step
+  CoarsePropagator(ctx, tn, DT)       T2S4_Parareal.cu:971  -> Context.coarse
+  FinePropagator(ctx, tn, DT, J)      T2S4_Parareal.cu:977  -> Context.fine
+  serial fine loop                    T2S4_Parareal.cu:1234 -> Context.plain_solve
+  Parareal iterations                 T2S4_Parareal.cu:1250 -> Context.parareal_solve
+
+Errors raise CcfvError carrying the C status code and message.  There is no
+CPU fallback: without the built library (or without a GPU) every call fails.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import types
+from .types import (IC, ParaReport, Problem, STATUS_NAMES, CCFV_OK,  # noqa: F401
+                    CCFV_ENOTCONV)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get(
+    "CCFV_LIB", os.path.join(os.path.dirname(_HERE), "lib", "libccfv.so"))
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+# every exported symbol of include/ccfv.h, with its ctypes signature
+_SIGS = {
+    "ccfv_last_error": (ctypes.c_char_p, []),
+    "ccfv_version": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32)] * 2),
+    "ccfv_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32)]),
+    "ccfv_problem_reference": (ctypes.c_int, [ctypes.POINTER(Problem),
+                                              ctypes.c_int32, ctypes.c_double]),
+    "ccfv_ic_reference": (ctypes.c_int, [ctypes.POINTER(IC), ctypes.c_double]),
+    "ccfv_create": (ctypes.c_int, [ctypes.POINTER(Problem),
+                                   ctypes.POINTER(ctypes.c_void_p)]),
+    "ccfv_destroy": (None, [ctypes.c_void_p]),
+    "ccfv_set_state": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "ccfv_get_state": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "ccfv_set_state_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "ccfv_get_state_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "ccfv_fill_exact": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(IC),
+                                       ctypes.c_double]),
+    "ccfv_exact_to_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(IC),
+                                          ctypes.c_double, _dp]),
+    "ccfv_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double,
+                                 ctypes.c_double]),
+    "ccfv_coarse": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double,
+                                   ctypes.c_double]),
+    "ccfv_fine": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_int32]),
+    "ccfv_plain_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_int32]),
+    "ccfv_parareal_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_int32,
+                                           ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_double,
+                                           ctypes.POINTER(ParaReport)]),
+    "ccfv_parareal_correction": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4
+                                 + [ctypes.c_size_t]),
+    "ccfv_sync": (ctypes.c_int, [ctypes.c_void_p]),
+}
+
+_lib = None
+
+
+class CcfvError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def load(path: str | None = None):
+    """Load lib/libccfv.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"libccfv.so not found at {p}: build it with "
+            "`make -C parareal-cc-cfv-split-gpu_amd/csrc` (or __graft_entry__.build())")
+    L = ctypes.CDLL(p)
+    for name, (res, args) in _SIGS.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _lib = L
+    return L
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def _check(rc: int, what: str):
+    if rc != CCFV_OK:
+        msg = load().ccfv_last_error().decode(errors="replace")
+        raise CcfvError(rc, f"{what}: {msg}")
+
+
+def device_count() -> int:
+    n = ctypes.c_int32(0)
+    rc = load().ccfv_device_count(ctypes.byref(n))
+    return n.value if rc == CCFV_OK else 0
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+class Context:
+    """One solver context on the current HIP device (ccfv_create)."""
+
+    def __init__(self, problem: Problem):
+        self.problem = problem.copy()
+        h = ctypes.c_void_p()
+        _check(load().ccfv_create(ctypes.byref(self.problem), ctypes.byref(h)),
+               "ccfv_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            load().ccfv_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def shape(self):
+        return (self.problem.nx, self.problem.ny)
+
+    def set_state(self, u: np.ndarray):
+        a = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        if a.size != self.problem.nx * self.problem.ny:
+            raise ValueError("state has the wrong size")
+        _check(load().ccfv_set_state(self._h, _ptr(a)), "ccfv_set_state")
+
+    def get_state(self) -> np.ndarray:
+        out = np.empty(self.problem.nx * self.problem.ny, dtype=np.float64)
+        _check(load().ccfv_get_state(self._h, _ptr(out)), "ccfv_get_state")
+        return out
+
+    def fill_exact(self, ic: IC, t: float = 0.0):
+        _check(load().ccfv_fill_exact(self._h, ctypes.byref(ic), t),
+               "ccfv_fill_exact")
+
+    def exact(self, ic: IC, t: float) -> np.ndarray:
+        out = np.empty(self.problem.nx * self.problem.ny, dtype=np.float64)
+        _check(load().ccfv_exact_to_host(self._h, ctypes.byref(ic), t, _ptr(out)),
+               "ccfv_exact_to_host")
+        return out
+
+    def step(self, tn: float, h: float):
+        _check(load().ccfv_step(self._h, tn, h), "ccfv_step")
+
+    def coarse(self, tn: float, DT: float):
+        _check(load().ccfv_coarse(self._h, tn, DT), "ccfv_coarse")
+
+    def fine(self, tn: float, DT: float, J: int):
+        _check(load().ccfv_fine(self._h, tn, DT, J), "ccfv_fine")
+
+    def plain_solve(self, t0: float, t1: float, nt: int):
+        _check(load().ccfv_plain_solve(self._h, t0, t1, nt), "ccfv_plain_solve")
+
+    def parareal_solve(self, t0, t1, nt_fine, n_slices, kmax, tol=0.0,
+                       raise_on_notconv=False) -> ParaReport:
+        rep = ParaReport()
+        rc = load().ccfv_parareal_solve(self._h, t0, t1, nt_fine, n_slices, kmax,
+                                        tol, ctypes.byref(rep))
+        if rc == CCFV_ENOTCONV and not raise_on_notconv:
+            return rep
+        _check(rc, "ccfv_parareal_solve")
+        return rep
+
+    def sync(self):
+        _check(load().ccfv_sync(self._h), "ccfv_sync")
+
+
+__all__ = ["Context", "CcfvError", "load", "device_count", "types", "Problem",
+           "IC", "ParaReport", "exported_symbols"]

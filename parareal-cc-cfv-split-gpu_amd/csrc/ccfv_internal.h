@@ -1,0 +1,89 @@
+// ccfv_internal.h — types shared by the gfx950 kernels and the host runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/ccfv.h"
+
+namespace ccfv {
+
+enum Dir { DIR_X = 0, DIR_Y = 1 };
+
+// Epilogue of an x sweep (the last sub-step of a coarse step inside the
+// Parareal chain).
+//   EPI_NONE   : store the swept field.
+//   EPI_INIT   : G cache <- g, U[n+1] <- g                 (k = 0 coarse sweep)
+//   EPI_CORRECT: U[n+1] <- g + (F - G_old), G cache <- g,  increment partials
+enum Epi { EPI_NONE = 0, EPI_INIT = 1, EPI_CORRECT = 2 };
+
+// Per-(direction, step size) constants of the implicit half: tridiagonal
+// coefficients and the Thomas factors of the constant matrix (identical for
+// every line of a sweep, so they are computed once on the host).
+struct SolveTable {
+    double *dev = nullptr;  // [mneg(n) | den(n) | pneg(n) | z(n)]
+    double ca = 0, cb = 0;  // off-diagonal, diagonal
+    double diag0 = 0;       // first diagonal entry of the factored matrix
+    double coef = 0;        // explicit diffusion factor (theta*nu)*(h/d)
+    double vl = 0, smden = 1;  // Sherman-Morrison (periodic)
+};
+
+// Everything one sweep kernel needs, passed by value.
+struct SweepArgs {
+    const double *in;   // field 0 input
+    double *out;        // field 0 output (may alias in: sweeps are line-local)
+    long long fstride;  // doubles between consecutive fields
+    int n;              // cells along a line
+    int nl;             // lines (cells in the other direction)
+    int ld;             // row length of the field (ny)
+    int quirk;          // reference ghost selection in the derivative stencil
+    double lo, hi, d;   // along-line box and cell width
+    double olo, od;     // other-direction box start and cell width
+    double eps;         // remap clip epsilon (closed box)
+    double coef, ca, diag0, vl, smden;
+    const double *tab;
+    int vel_kind;
+    double v0, v1, v2;  // velocity parameters
+    // time: field f of the launch sits at tn = t0 + (field0 + f)*tDT, the
+    // sub-step starts at tl = tn + jdt and the velocity is sampled at
+    // [t_lo, t_hi] of the sub-step (only time-dependent velocities read it)
+    double t0, tDT, jdt, h;
+    int field0, substep;
+    double tau;
+    int *flag;          // device status word (set to 1 on crossed feet)
+    // epilogue (x sweeps inside the Parareal coarse chain)
+    double *g_cache;      // G(U_n) cache slot
+    const double *f_in;   // F(U_n^k)
+    double *u_next;       // U[n+1] (read old, write new)
+    double *partial;      // 2 doubles per block: sum (du)^2, sum u^2
+};
+
+struct Tile {
+    int w;   // lines per block
+    int nw;  // wavefronts per block
+};
+
+// cells per lane for a line of n cells: smallest power of two with 64*ch >= n
+int chunk_for(int n);
+bool supported_n(int n);
+
+// kernels (ccfv_kernels.hip)
+hipError_t launch_sweep(Dir dir, int bc, bool post, Epi epi, const SweepArgs &a,
+                        int nfields, Tile tile, hipStream_t s);
+size_t sweep_lds_bytes(int n, Tile tile, bool post);
+hipError_t launch_correction(const double *g_new, const double *g_old,
+                             const double *f, double *u, size_t n,
+                             double *partial, int nblocks, hipStream_t s);
+hipError_t launch_reduce_partials(const double *partial, int count,
+                                  double *out2, hipStream_t s);
+// per-slot sums of `seg` (num, den) partial pairs, slots first..first+count-1
+hipError_t launch_reduce_segments(const double *partial, int seg, int first,
+                                  int count, double *out, hipStream_t s);
+hipError_t launch_fill_exact(const ccfv_problem &p, const ccfv_ic &ic, double t,
+                             double *out, hipStream_t s);
+hipError_t launch_copy(const double *src, double *dst, size_t n, hipStream_t s);
+hipError_t launch_nonfinite_check(const double *src, size_t n, int *flag,
+                                  hipStream_t s);
+
+}  // namespace ccfv

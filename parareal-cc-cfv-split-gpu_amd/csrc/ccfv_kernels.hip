@@ -1,0 +1,978 @@
+// ccfv_kernels.hip — gfx950 kernels for the T2S4 CC-CFV line sweeps.
+//
+// One kernel launch performs one whole sub-step of the Strang step
+// (T2S4_Parareal.cu:825-870, :873-917 or :920-963) for a batch of fields: a
+// block stages a tile of W lines in LDS, each wavefront sweeps whole lines
+// (64 lanes, `CH` contiguous cells per lane) and the tile is written back in
+// place.  Everything the reference keeps in 30+ global scratch arrays
+// (feet, PPM coefficients, remap, foot derivatives, RHS, compact RHS, Thomas
+// p/q) lives in registers and LDS, so a sweep moves 8 B in + 8 B out per cell.
+//
+// Per line (device function process_line):
+//   1. feet of the n+1 faces by the Heun trace of the two corners of each
+//      face (:203-258), computed without fused multiply-add so that they are
+//      bit-identical to the CPU restatement (the cell a foot lands in is a
+//      discrete decision);
+//   2. per face: the foot's cell and offset, the PPM parabola of that cell
+//      (:260-304), its antiderivative at the foot (:63-65) and the quartic
+//      derivative at the foot (:440-547);
+//   3. per cell: the conservative remap between its two feet (:306-347),
+//      the RHS with the explicit diffusion half (:564-613) and the compact
+//      (1,10,1)/12 operator (:616-654), the neighbours across lanes by shuffle;
+//   4. the tridiagonal implicit half (:657-719) as two first-order affine
+//      recurrences (the Thomas forward and backward passes, with the LU
+//      factors of the constant matrix precomputed on the host), each solved
+//      by a per-lane chunk pass plus a 64-lane Hillis-Steele scan; periodic
+//      lines add one Sherman-Morrison correction.
+#include "ccfv_internal.h"
+
+#include <cmath>
+
+namespace ccfv {
+
+constexpr int BC_REF = CCFV_BC_REFERENCE;
+constexpr int BC_PER = CCFV_BC_PERIODIC;
+
+// LDS position of element e of a line: one pad slot every 8 elements keeps
+// both the chunked (lane*CH + k) and the strided (lane + 64k) access
+// patterns free of bank conflicts.
+__device__ __forceinline__ int pidx(int e) { return e + (e >> 3); }
+__host__ __device__ inline int line_pitch(int n) { return n + (n >> 3) + 1; }
+
+int chunk_for(int n)
+{
+    int ch = 1;
+    while (ch * 64 < n) ch *= 2;
+    return ch;
+}
+
+bool supported_n(int n)
+{
+    if (n < 8 || n > 2048 || (n & 1)) return false;
+    return n % chunk_for(n) == 0;
+}
+
+// ---------------------------------------------------------------------------
+// feet (no contraction: identical rounding to the CPU restatement)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double vel_x(const SweepArgs &a, double x, double y,
+                                        double t)
+{
+#pragma clang fp contract(off)
+    if (a.vel_kind == CCFV_VEL_SOLID_BODY) return -a.v0 * (y - a.v2);
+    if (a.vel_kind == CCFV_VEL_CONSTANT) return a.v0;
+    double s = sin(M_PI * x);
+    return s * s * sin(2.0 * M_PI * y) * cos(M_PI * t / a.v0);
+}
+
+__device__ __forceinline__ double vel_y(const SweepArgs &a, double x, double y,
+                                        double t)
+{
+#pragma clang fp contract(off)
+    if (a.vel_kind == CCFV_VEL_SOLID_BODY) return a.v0 * (x - a.v1);
+    if (a.vel_kind == CCFV_VEL_CONSTANT) return a.v1;
+    double s = sin(M_PI * y);
+    return -(s * s) * sin(2.0 * M_PI * x) * cos(M_PI * t / a.v0);
+}
+
+// Foot of face q of a line: mean of the Heun feet of its two corners
+// (T2S4_Parareal.cu:215-225, :240, :255).
+struct Times {
+    double hi, lo;
+};
+
+// sampling times of the velocity for the sub-step of field blockIdx.y:
+// x sub-step 1 over [tn, tn+h/2], y over [tn, tn+h], x sub-step 3 over
+// [tn+h/2, tn+h]; tn = t0 + n*DT (+ j*dt inside the fine propagator, :981)
+__device__ __forceinline__ Times sub_times(const SweepArgs &a)
+{
+#pragma clang fp contract(off)
+    double tn = a.t0 + (double)(a.field0 + (int)blockIdx.y) * a.tDT;
+    double tl = tn + a.jdt;
+    Times t;
+    if (a.substep == 1) {
+        t.lo = tl;
+        t.hi = tl + 0.5 * a.h;
+    } else if (a.substep == 3) {
+        t.lo = tl + 0.5 * a.h;
+        t.hi = tl + a.h;
+    } else {
+        t.lo = tl;
+        t.hi = tl + a.h;
+    }
+    return t;
+}
+
+template <int DIR>
+__device__ __forceinline__ double face_foot(const SweepArgs &a, int q,
+                                            double o0, double o1)
+{
+#pragma clang fp contract(off)
+    double s = a.lo + a.d * q;
+    double ht = 0.5 * a.tau;
+    Times tt = sub_times(a);
+    if (DIR == DIR_X) {
+        double v0 = vel_x(a, s, o0, tt.hi);
+        double v1 = vel_x(a, s - a.tau * v0, o0, tt.lo);
+        double f0 = s - ht * (v0 + v1);
+        double w0 = vel_x(a, s, o1, tt.hi);
+        double w1 = vel_x(a, s - a.tau * w0, o1, tt.lo);
+        double f1 = s - ht * (w0 + w1);
+        return 0.5 * (f0 + f1);
+    } else {
+        double v0 = vel_y(a, o0, s, tt.hi);
+        double v1 = vel_y(a, o0, s - a.tau * v0, tt.lo);
+        double f0 = s - ht * (v0 + v1);
+        double w0 = vel_y(a, o1, s, tt.hi);
+        double w1 = vel_y(a, o1, s - a.tau * w0, tt.lo);
+        double f1 = s - ht * (w0 + w1);
+        return 0.5 * (f0 + f1);
+    }
+}
+
+// cell of position x and the offset in it, (x - lo)/d floored (:325-328)
+__device__ __forceinline__ int cell_of(const SweepArgs &a, double x,
+                                       double &xi)
+{
+#pragma clang fp contract(off)
+    double v = floor((x - a.lo) / a.d);
+    v = fmin(fmax(v, -4.0 * a.n), 4.0 * a.n);  // keep the int conversion defined
+    int m = (int)v;
+    xi = (x - (a.lo + a.d * m)) / a.d;
+    return m;
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi)
+{
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// ---------------------------------------------------------------------------
+// reconstruction
+// ---------------------------------------------------------------------------
+// zero-slope cubic ghosts {Cm2, Cm1, CN, CNp1} (T2S4_Parareal.cu:275-286)
+__device__ __forceinline__ void ghosts(const double *cs, int n, double g[4])
+{
+    const double e = 1.0 / 11.0;
+    double c0 = cs[pidx(0)], c1 = cs[pidx(1)], c2 = cs[pidx(2)];
+    double a0 = cs[pidx(n - 3)], a1 = cs[pidx(n - 2)], a2 = cs[pidx(n - 1)];
+    g[2] = e * (-a0 + 3 * a1 + 9 * a2);
+    g[3] = e * (-15 * a0 + 56 * a1 - 30 * a2);
+    g[1] = e * (-c2 + 3 * c1 + 9 * c0);
+    g[0] = e * (-15 * c2 + 56 * c1 - 30 * c0);
+}
+
+// five cell averages around cell m: E[m-2..m+2]; m in [0,n) (periodic: m is
+// already wrapped; the closed box substitutes the ghosts)
+template <int BC>
+__device__ __forceinline__ void window(const double *cs, int n,
+                                       const double g[4], int m, double w[5])
+{
+#pragma unroll
+    for (int s = 0; s < 5; s++) {
+        int k = m - 2 + s;
+        if (BC == BC_PER) {
+            k = k < 0 ? k + n : (k >= n ? k - n : k);
+            w[s] = cs[pidx(k)];
+        } else {
+            double v = cs[pidx(clampi(k, 0, n - 1))];
+            w[s] = k < 0 ? (k == -1 ? g[1] : g[0])
+                         : (k >= n ? (k == n ? g[2] : g[3]) : v);
+        }
+    }
+}
+
+struct Ppm {
+    double cl, dl, c6;
+};
+
+// PPM parabola of the centre cell of w (T2S4_Parareal.cu:296-300)
+__device__ __forceinline__ Ppm ppm(const double w[5])
+{
+    const double s12 = 7.0 / 12.0, o12 = 1.0 / 12.0;
+    double cl = s12 * (w[2] + w[1]) - o12 * (w[3] + w[0]);
+    double cr = s12 * (w[3] + w[2]) - o12 * (w[4] + w[1]);
+    Ppm p;
+    p.cl = cl;
+    p.dl = cr - cl;
+    p.c6 = 6.0 * (w[2] - 0.5 * (cl + cr));
+    return p;
+}
+
+// antiderivative of the parabola (T2S4_Parareal.cu:63-65)
+__device__ __forceinline__ double intc(const Ppm &p, double xi)
+{
+    return -1.0 / 3 * p.c6 * (xi * xi * xi) + 0.5 * (p.dl + p.c6) * (xi * xi) +
+           p.cl * xi;
+}
+
+// derivative of the quartic through the five averages (T2S4_Parareal.cu:481-486)
+__device__ __forceinline__ double deriv(const double w[5], double xi,
+                                        double inv12d)
+{
+    double x2 = xi * xi, x3 = x2 * xi;
+    return inv12d * ((2.0 * x3 - 6.0 * x2 + 3.0 * xi + 1.0) * w[0] +
+                     (-8.0 * x3 + 18.0 * x2 + 6.0 * xi - 15.0) * w[1] +
+                     (12.0 * x3 - 18.0 * x2 - 24.0 * xi + 15.0) * w[2] +
+                     (-8.0 * x3 + 6.0 * x2 + 18.0 * xi - 1.0) * w[3] +
+                     (2.0 * x3 - 3.0 * xi) * w[4]);
+}
+
+struct Face {
+    double xb;        // raw foot
+    double xiL;       // offset of the (left-clipped) foot in its cell
+    int mL, mR;       // cells of the foot as left / right end of a cell
+    double ivL, icL;  // I(xi), I(1)-I(xi) as left end
+    double ivR;       // I(xi) as right end
+    double D;         // derivative at the foot
+};
+
+// Closed box foot derivative: zero outside the box, else clamp to
+// [lo+1e-15, hi-1e-15] (T2S4_Parareal.cu:457-490, :514-543).
+template <int BC>
+__device__ __forceinline__ double deriv_ref(const SweepArgs &a,
+                                            const double *src,
+                                            const double g[4], double xb,
+                                            double inv12d)
+{
+#pragma clang fp contract(off)
+    if (!(xb >= a.lo && xb <= a.hi)) return 0.0;
+    double xd = fmin(fmax(xb, a.lo + 1e-15), a.hi - 1e-15);
+    double xi;
+    int m = cell_of(a, xd, xi);
+    int md = clampi(m, 0, a.n - 1);
+    if (md != m) xi = (xd - (a.lo + a.d * md)) / a.d;
+    double w[5];
+    window<BC_REF>(src, a.n, g, md, w);
+    if (a.quirk) {  // :476-479 — iL==Nx-2 takes CNp1, iL==1 takes Cm2
+        if (md > a.n - 3) w[4] = g[3];
+        if (md < 2) w[0] = g[0];
+    }
+    return deriv(w, xi, inv12d);
+}
+
+template <int DIR, int BC, bool POST>
+__device__ __forceinline__ Face eval_face(const SweepArgs &a, const double *cs,
+                                          const double g[4], int q, double o0,
+                                          double o1, double inv12d)
+{
+    Face f;
+    const int n = a.n;
+    if (BC == BC_PER) {
+        // face n is face 0 one period on (exact periodicity of the fluxes)
+        int qq = q == n ? 0 : q;
+        f.xb = face_foot<DIR>(a, qq, o0, o1);
+        double xi;
+        int m = cell_of(a, f.xb, xi);
+        int mw = m % n;
+        mw = mw < 0 ? mw + n : mw;
+        double w[5];
+        window<BC_PER>(cs, n, g, mw, w);
+        Ppm p = ppm(w);
+        double iv = intc(p, xi);
+        f.ivL = f.ivR = iv;
+        f.icL = intc(p, 1.0) - iv;
+        f.xiL = xi;
+        f.mL = f.mR = q == n ? m + n : m;
+        f.D = POST ? 0.0 : deriv(w, xi, inv12d);
+    } else {
+        f.xb = face_foot<DIR>(a, q, o0, o1);
+        double xl, xr;
+        {
+#pragma clang fp contract(off)
+            xl = fmax(f.xb, a.lo + a.eps);
+            xr = fmin(f.xb, a.hi - a.eps);
+        }
+        double xi;
+        int m = clampi(cell_of(a, xl, xi), 0, n - 1);
+        {
+#pragma clang fp contract(off)
+            xi = (xl - (a.lo + a.d * m)) / a.d;
+        }
+        double w[5];
+        window<BC_REF>(cs, n, g, m, w);
+        Ppm p = ppm(w);
+        f.mL = m;
+        f.xiL = xi;
+        f.ivL = intc(p, xi);
+        f.icL = intc(p, 1.0) - f.ivL;
+        if (xr == xl) {
+            f.mR = m;
+            f.ivR = f.ivL;
+        } else {
+            double xi2;
+            int m2 = clampi(cell_of(a, xr, xi2), 0, n - 1);
+            {
+#pragma clang fp contract(off)
+                xi2 = (xr - (a.lo + a.d * m2)) / a.d;
+            }
+            double w2[5];
+            window<BC_REF>(cs, n, g, m2, w2);
+            Ppm p2 = ppm(w2);
+            f.mR = m2;
+            f.ivR = intc(p2, xi2);
+        }
+        f.D = POST ? 0.0 : deriv_ref<BC>(a, cs, g, f.xb, inv12d);
+    }
+    return f;
+}
+
+// conservative remap of one cell between its feet (T2S4_Parareal.cu:319-344)
+template <int BC>
+__device__ __forceinline__ double remap_cell(const SweepArgs &a,
+                                             const double *cs, const Face &L,
+                                             const Face &R, bool &bad)
+{
+    const int n = a.n;
+    if (BC == BC_REF) {
+        if (R.xb < L.xb) bad = true;
+        if (L.xb > a.hi || R.xb < a.lo || L.xb == R.xb) return 0.0;
+        if (R.mR == L.mL) return R.ivR - L.ivL;
+        double s = 0.0;
+        for (int m = L.mL + 1; m <= R.mR - 1; m++) s += cs[pidx(m)];
+        return L.icL + s + R.ivR;
+    } else {
+        int mL = L.mL, mR = R.mR;
+        if (mR < mL || (mR == mL && R.xiL < L.xiL) || mR - mL > n) {
+            bad = true;
+            return 0.0;
+        }
+        if (mR == mL) return R.ivR - L.ivL;
+        int w = (mL + 1) % n;
+        w = w < 0 ? w + n : w;
+        double s = 0.0;
+        for (int m = mL + 1; m <= mR - 1; m++) {
+            s += cs[pidx(w)];
+            w = w + 1 == n ? 0 : w + 1;
+        }
+        return L.icL + s + R.ivR;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// one line, one wavefront
+// ---------------------------------------------------------------------------
+// cs: the line in LDS (pidx layout), overwritten with the result.
+// aux: a line-sized LDS buffer of this wave (POST only).
+template <int DIR, int BC, int CH, bool POST>
+__device__ void process_line(const SweepArgs &a, double *cs, double *aux,
+                             int line, int lane)
+{
+    const int n = a.n;
+    const int NL = n / CH;
+    const bool active = lane < NL;
+    const int base = lane * CH;
+    const double inv12d = 1.0 / (12.0 * a.d);
+    double o0, o1;
+    {
+#pragma clang fp contract(off)
+        o0 = a.olo + a.od * line;
+        o1 = a.olo + a.od * (line + 1);
+    }
+    double g[4] = {0.0, 0.0, 0.0, 0.0};
+    if (BC == BC_REF) ghosts(cs, n, g);
+
+    double b[CH];
+    bool bad = false;
+    if (active) {
+        Face fl = eval_face<DIR, BC, POST>(a, cs, g, base, o0, o1, inv12d);
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            Face fr = eval_face<DIR, BC, POST>(a, cs, g, base + k + 1, o0, o1,
+                                               inv12d);
+            double cbar = remap_cell<BC>(a, cs, fl, fr, bad);
+            b[k] = POST ? cbar : cbar + a.coef * (fr.D - fl.D);
+            fl = fr;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < CH; k++) b[k] = 0.0;
+    }
+    if (bad) *a.flag = 1;
+
+    if (POST) {
+        // the reference's third sub-step takes the derivative of the remapped
+        // line (T2S4_Parareal.cu:929-932)
+        if (active) {
+#pragma unroll
+            for (int k = 0; k < CH; k++) aux[pidx(base + k)] = b[k];
+        }
+        __syncthreads();
+        double gs[4] = {0.0, 0.0, 0.0, 0.0};
+        if (BC == BC_REF) ghosts(aux, n, gs);
+        if (active) {
+            double dl;
+            if (BC == BC_REF) {
+                dl = deriv_ref<BC>(a, aux, gs, face_foot<DIR>(a, base, o0, o1),
+                                   inv12d);
+            } else {
+                Face f0 = eval_face<DIR, BC, false>(a, aux, gs, base, o0, o1,
+                                                    inv12d);
+                dl = f0.D;
+            }
+#pragma unroll
+            for (int k = 0; k < CH; k++) {
+                double dr;
+                if (BC == BC_REF) {
+                    dr = deriv_ref<BC>(a, aux, gs,
+                                       face_foot<DIR>(a, base + k + 1, o0, o1),
+                                       inv12d);
+                } else {
+                    Face f1 = eval_face<DIR, BC, false>(a, aux, gs, base + k + 1,
+                                                        o0, o1, inv12d);
+                    dr = f1.D;
+                }
+                b[k] = b[k] + a.coef * (dr - dl);
+                dl = dr;
+            }
+        }
+        __syncthreads();
+    }
+
+    // compact operator (1, 10, 1)/12 (T2S4_Parareal.cu:616-654)
+    double bprev = __shfl_up(b[CH - 1], 1);
+    double bnext = __shfl_down(b[0], 1);
+    double wrap_last = __shfl(b[CH - 1], NL - 1);
+    double wrap_first = __shfl(b[0], 0);
+    if (lane == 0) bprev = BC == BC_PER ? wrap_last : 0.0;
+    if (lane == NL - 1) bnext = BC == BC_PER ? wrap_first : 0.0;
+    double r[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        double bn = k + 1 < CH ? b[k + 1] : bnext;
+        double bp = k > 0 ? b[k - 1] : bprev;
+        r[k] = (bn + 10.0 * b[k] + bp) / 12.0;
+    }
+
+    // forward recurrence q_e = c_e + m_e q_{e-1} (Thomas forward pass)
+    const double *mneg = a.tab, *den = a.tab + n, *pneg = a.tab + 2 * n,
+                 *zt = a.tab + 3 * n;
+    double M = 1.0, Q = 0.0;
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            int e = base + k;
+            double c = e == 0 ? r[k] / a.diag0 : r[k] * den[e];
+            double m = mneg[e];
+            r[k] = c;
+            Q = c + m * Q;
+            M = m * M;
+        }
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        double Mp = __shfl_up(M, off), Qp = __shfl_up(Q, off);
+        if (lane >= off) {
+            Q = M * Qp + Q;
+            M = M * Mp;
+        }
+    }
+    double carry = __shfl_up(Q, 1);
+    if (lane == 0) carry = 0.0;
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            carry = r[k] + mneg[base + k] * carry;
+            r[k] = carry;
+        }
+    }
+
+    // backward recurrence x_e = q_e + pneg_e x_{e+1} (back substitution)
+    M = 1.0;
+    Q = 0.0;
+    if (active) {
+#pragma unroll
+        for (int k = CH - 1; k >= 0; k--) {
+            double pn = pneg[base + k];
+            Q = r[k] + pn * Q;
+            M = pn * M;
+        }
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        double Mn = __shfl_down(M, off), Qn = __shfl_down(Q, off);
+        if (lane + off < 64) {
+            Q = M * Qn + Q;
+            M = M * Mn;
+        }
+    }
+    carry = __shfl_down(Q, 1);
+    if (lane == 63) carry = 0.0;
+    if (active) {
+#pragma unroll
+        for (int k = CH - 1; k >= 0; k--) {
+            carry = r[k] + pneg[base + k] * carry;
+            r[k] = carry;
+        }
+    }
+
+    if (BC == BC_PER) {
+        // Sherman-Morrison: x = y - ((y_0 + vl y_{n-1}) / (1 + v.z)) z
+        double y0 = __shfl(r[0], 0);
+        double yn = __shfl(r[CH - 1], NL - 1);
+        double fac = (y0 + a.vl * yn) / a.smden;
+        if (active) {
+#pragma unroll
+            for (int k = 0; k < CH; k++) r[k] = r[k] - fac * zt[base + k];
+        }
+    }
+
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) cs[pidx(base + k)] = r[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// block reduction of two doubles into partial[2*blockIdx.x ...]
+// ---------------------------------------------------------------------------
+__device__ void block_reduce2(double s0, double s1, double *partial, int slot)
+{
+    __shared__ double red[2][16];
+    for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_down(s0, off);
+        s1 += __shfl_down(s1, off);
+    }
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        red[0][wave] = s0;
+        red[1][wave] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t0 = 0.0, t1 = 0.0;
+        int nw = blockDim.x >> 6;
+        for (int w = 0; w < nw; w++) {
+            t0 += red[0][w];
+            t1 += red[1][w];
+        }
+        partial[2 * slot] = t0;
+        partial[2 * slot + 1] = t1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// sweep kernels
+// ---------------------------------------------------------------------------
+// x sweep: lines are columns j of the row-major field (stride ny).  A block
+// owns W adjacent columns: the tile rows (W doubles each) are read and written
+// coalesced and transposed through LDS.
+template <int BC, int CH, bool POST, int EPI>
+__global__ __launch_bounds__(512) void sweep_x_kernel(SweepArgs a, int W, int NW)
+{
+    extern __shared__ double lds[];
+    const int n = a.n, LP = line_pitch(n);
+    const size_t foff = (size_t)blockIdx.y * (size_t)a.fstride;
+    const double *in = a.in + foff;
+    double *out = a.out + foff;
+    const int j0 = blockIdx.x * W;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int hw = W >> 1;  // double2 per row
+
+    for (int idx = tid; idx < hw * n; idx += nth) {
+        int c2 = idx % hw, i = idx / hw;
+        double2 v = *reinterpret_cast<const double2 *>(in + (size_t)i * a.ld + j0 +
+                                                       2 * c2);
+        lds[(2 * c2) * LP + pidx(i)] = v.x;
+        lds[(2 * c2 + 1) * LP + pidx(i)] = v.y;
+    }
+    __syncthreads();
+    const int wave = tid >> 6, lane = tid & 63;
+    double *aux = lds + W * LP + wave * LP;
+    for (int l = wave; l < W; l += NW)
+        process_line<DIR_X, BC, CH, POST>(a, lds + l * LP, aux, j0 + l, lane);
+    __syncthreads();
+
+    if (EPI == EPI_NONE) {
+        for (int idx = tid; idx < hw * n; idx += nth) {
+            int c2 = idx % hw, i = idx / hw;
+            double2 v;
+            v.x = lds[(2 * c2) * LP + pidx(i)];
+            v.y = lds[(2 * c2 + 1) * LP + pidx(i)];
+            *reinterpret_cast<double2 *>(out + (size_t)i * a.ld + j0 + 2 * c2) = v;
+        }
+    } else {
+        double s0 = 0.0, s1 = 0.0;
+        for (int idx = tid; idx < hw * n; idx += nth) {
+            int c2 = idx % hw, i = idx / hw;
+            size_t gi = (size_t)i * a.ld + j0 + 2 * c2;
+            double2 g;
+            g.x = lds[(2 * c2) * LP + pidx(i)];
+            g.y = lds[(2 * c2 + 1) * LP + pidx(i)];
+            if (EPI == EPI_INIT) {
+                *reinterpret_cast<double2 *>(a.g_cache + gi) = g;
+                *reinterpret_cast<double2 *>(a.u_next + gi) = g;
+            } else {
+                // U_{n+1} = G_new + (F - G_old)  (T2S4_Parareal.cu:779)
+                double2 go = *reinterpret_cast<const double2 *>(a.g_cache + gi);
+                double2 fv = *reinterpret_cast<const double2 *>(a.f_in + gi);
+                double2 uo = *reinterpret_cast<const double2 *>(a.u_next + gi);
+                double2 un;
+                un.x = g.x + (fv.x - go.x);
+                un.y = g.y + (fv.y - go.y);
+                *reinterpret_cast<double2 *>(a.g_cache + gi) = g;
+                *reinterpret_cast<double2 *>(a.u_next + gi) = un;
+                double dx0 = un.x - uo.x, dx1 = un.y - uo.y;
+                s0 += dx0 * dx0 + dx1 * dx1;
+                s1 += un.x * un.x + un.y * un.y;
+            }
+        }
+        if (EPI == EPI_CORRECT) block_reduce2(s0, s1, a.partial, blockIdx.x);
+    }
+}
+
+// y sweep: lines are rows i of the field (contiguous).  A block owns W rows.
+template <int BC, int CH>
+__global__ __launch_bounds__(512) void sweep_y_kernel(SweepArgs a, int W, int NW)
+{
+    extern __shared__ double lds[];
+    const int n = a.n, LP = line_pitch(n);
+    const size_t foff = (size_t)blockIdx.y * (size_t)a.fstride;
+    const int i0 = blockIdx.x * W;
+    const double *in = a.in + foff + (size_t)i0 * a.ld;
+    double *out = a.out + foff + (size_t)i0 * a.ld;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int hn = n >> 1;
+
+    for (int idx = tid; idx < W * hn; idx += nth) {
+        int r = idx / hn, j2 = idx - r * hn;
+        double2 v = *reinterpret_cast<const double2 *>(in + (size_t)r * a.ld + 2 * j2);
+        lds[r * LP + pidx(2 * j2)] = v.x;
+        lds[r * LP + pidx(2 * j2 + 1)] = v.y;
+    }
+    __syncthreads();
+    const int wave = tid >> 6, lane = tid & 63;
+    for (int l = wave; l < W; l += NW)
+        process_line<DIR_Y, BC, CH, false>(a, lds + l * LP, nullptr, i0 + l, lane);
+    __syncthreads();
+    for (int idx = tid; idx < W * hn; idx += nth) {
+        int r = idx / hn, j2 = idx - r * hn;
+        double2 v;
+        v.x = lds[r * LP + pidx(2 * j2)];
+        v.y = lds[r * LP + pidx(2 * j2 + 1)];
+        *reinterpret_cast<double2 *>(out + (size_t)r * a.ld + 2 * j2) = v;
+    }
+}
+
+size_t sweep_lds_bytes(int n, Tile tile, bool post)
+{
+    return sizeof(double) * (size_t)line_pitch(n) *
+           (size_t)(tile.w + (post ? tile.nw : 0));
+}
+
+template <int BC, int CH, bool POST, int EPI>
+static hipError_t go_x(const SweepArgs &a, int nfields, Tile t, hipStream_t s)
+{
+    dim3 grid(a.nl / t.w, nfields), block(64 * t.nw);
+    size_t lds = sweep_lds_bytes(a.n, t, POST);
+    auto k = sweep_x_kernel<BC, CH, POST, EPI>;
+    hipError_t e = hipFuncSetAttribute((const void *)k,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, grid, block, lds, s, a, t.w, t.nw);
+    return hipGetLastError();
+}
+
+template <int BC, int CH>
+static hipError_t go_y(const SweepArgs &a, int nfields, Tile t, hipStream_t s)
+{
+    dim3 grid(a.nl / t.w, nfields), block(64 * t.nw);
+    size_t lds = sweep_lds_bytes(a.n, t, false);
+    auto k = sweep_y_kernel<BC, CH>;
+    hipError_t e = hipFuncSetAttribute((const void *)k,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, grid, block, lds, s, a, t.w, t.nw);
+    return hipGetLastError();
+}
+
+template <int BC, int CH>
+static hipError_t dispatch_x(bool post, Epi epi, const SweepArgs &a, int nf,
+                             Tile t, hipStream_t s)
+{
+    if (BC == BC_REF && post) {
+        if (epi == EPI_NONE) return go_x<BC, CH, true, EPI_NONE>(a, nf, t, s);
+        if (epi == EPI_INIT) return go_x<BC, CH, true, EPI_INIT>(a, nf, t, s);
+        return go_x<BC, CH, true, EPI_CORRECT>(a, nf, t, s);
+    }
+    if (epi == EPI_NONE) return go_x<BC, CH, false, EPI_NONE>(a, nf, t, s);
+    if (epi == EPI_INIT) return go_x<BC, CH, false, EPI_INIT>(a, nf, t, s);
+    return go_x<BC, CH, false, EPI_CORRECT>(a, nf, t, s);
+}
+
+template <int BC>
+static hipError_t dispatch_bc(Dir dir, bool post, Epi epi, const SweepArgs &a,
+                              int nf, Tile t, hipStream_t s)
+{
+    int ch = chunk_for(a.n);
+    if (dir == DIR_X) {
+        switch (ch) {
+        case 1: return dispatch_x<BC, 1>(post, epi, a, nf, t, s);
+        case 2: return dispatch_x<BC, 2>(post, epi, a, nf, t, s);
+        case 4: return dispatch_x<BC, 4>(post, epi, a, nf, t, s);
+        case 8: return dispatch_x<BC, 8>(post, epi, a, nf, t, s);
+        case 16: return dispatch_x<BC, 16>(post, epi, a, nf, t, s);
+        case 32: return dispatch_x<BC, 32>(post, epi, a, nf, t, s);
+        }
+    } else {
+        switch (ch) {
+        case 1: return go_y<BC, 1>(a, nf, t, s);
+        case 2: return go_y<BC, 2>(a, nf, t, s);
+        case 4: return go_y<BC, 4>(a, nf, t, s);
+        case 8: return go_y<BC, 8>(a, nf, t, s);
+        case 16: return go_y<BC, 16>(a, nf, t, s);
+        case 32: return go_y<BC, 32>(a, nf, t, s);
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_sweep(Dir dir, int bc, bool post, Epi epi, const SweepArgs &a,
+                        int nfields, Tile tile, hipStream_t s)
+{
+    if (nfields <= 0) return hipSuccess;
+    if (tile.w <= 0 || tile.nw <= 0 || tile.w % tile.nw || a.nl % tile.w ||
+        (tile.w & 1) && dir == DIR_X)
+        return hipErrorInvalidValue;
+    if (bc == BC_PER) return dispatch_bc<BC_PER>(dir, post, epi, a, nfields, tile, s);
+    return dispatch_bc<BC_REF>(dir, post, epi, a, nfields, tile, s);
+}
+
+// ---------------------------------------------------------------------------
+// element-wise kernels
+// ---------------------------------------------------------------------------
+// U = G_new + (F - G_old), with increment partials (T2S4_Parareal.cu:771-782)
+__global__ __launch_bounds__(256) void correction_kernel(
+    const double *g_new, const double *g_old, const double *f, double *u,
+    size_t n, double *partial)
+{
+    double s0 = 0.0, s1 = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        double un = g_new[i] + (f[i] - g_old[i]);
+        if (partial) {
+            double d = un - u[i];
+            s0 += d * d;
+            s1 += un * un;
+        }
+        u[i] = un;
+    }
+    if (partial) block_reduce2(s0, s1, partial, blockIdx.x);
+}
+
+hipError_t launch_correction(const double *g_new, const double *g_old,
+                             const double *f, double *u, size_t n,
+                             double *partial, int nblocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(correction_kernel, dim3(nblocks), dim3(256), 0, s, g_new,
+                       g_old, f, u, n, partial);
+    return hipGetLastError();
+}
+
+// fixed-order sum of count (num, den) pairs
+__global__ __launch_bounds__(256) void reduce_partials_kernel(
+    const double *partial, int count, double *out2)
+{
+    __shared__ double r0[256], r1[256];
+    double s0 = 0.0, s1 = 0.0;
+    for (int i = threadIdx.x; i < count; i += 256) {
+        s0 += partial[2 * i];
+        s1 += partial[2 * i + 1];
+    }
+    r0[threadIdx.x] = s0;
+    r1[threadIdx.x] = s1;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            r0[threadIdx.x] += r0[threadIdx.x + w];
+            r1[threadIdx.x] += r1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out2[0] = r0[0];
+        out2[1] = r1[0];
+    }
+}
+
+hipError_t launch_reduce_partials(const double *partial, int count, double *out2,
+                                  hipStream_t s)
+{
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, partial,
+                       count, out2);
+    return hipGetLastError();
+}
+
+// one thread per slot sums its segment in a fixed order
+__global__ void reduce_segments_kernel(const double *partial, int seg, int first,
+                                       int count, double *out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    int s = first + t;
+    const double *p = partial + 2 * (size_t)seg * s;
+    double s0 = 0.0, s1 = 0.0;
+    for (int i = 0; i < seg; i++) {
+        s0 += p[2 * i];
+        s1 += p[2 * i + 1];
+    }
+    out[2 * s] = s0;
+    out[2 * s + 1] = s1;
+}
+
+hipError_t launch_reduce_segments(const double *partial, int seg, int first,
+                                  int count, double *out, hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(reduce_segments_kernel, dim3((count + 63) / 64), dim3(64),
+                       0, s, partial, seg, first, count, out);
+    return hipGetLastError();
+}
+
+__global__ void copy_kernel(const double *src, double *dst, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy(const double *src, double *dst, size_t n, hipStream_t s)
+{
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
+__global__ void nonfinite_kernel(const double *src, size_t n, int *flag)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        if (!isfinite(src[i])) *flag = 1;
+}
+
+hipError_t launch_nonfinite_check(const double *src, size_t n, int *flag,
+                                  hipStream_t s)
+{
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(nonfinite_kernel, dim3(blocks), dim3(256), 0, s, src, n,
+                       flag);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// initial / exact cell averages (T2S4_Parareal.cu:118-140)
+// ---------------------------------------------------------------------------
+struct ExactArgs {
+    int nx, ny, bc, kind, ng;
+    double x_lo, y_lo, dx, dy, Lx, Ly, nu, t;
+    double p[16];
+};
+
+__device__ double rot_gauss(const ExactArgs &e, double x, double y)
+{
+    double x0 = e.p[0], y0 = e.p[1], s = e.p[2], om = e.p[3], cx = e.p[4],
+           cy = e.p[5], t = e.t;
+    double co = cos(om * t), si = sin(om * t);
+    double ax = x - cx, ay = y - cy;
+    double xs = cx + (ax * co + ay * si);
+    double ys = cy + (-ax * si + ay * co);
+    double den = s + 4 * e.nu * t;
+    double ex = xs - x0, ey = ys - y0;
+    return s / den * exp(-(ex * ex + ey * ey) / den);
+}
+
+__device__ double gaussians(const ExactArgs &e, double x, double y)
+{
+    double u = e.p[0], v = e.p[1], nu = e.p[2], t = e.t, acc = 0.0;
+    for (int k = 0; k < e.ng && k < 3; k++) {
+        const double *g = e.p + 3 + 4 * k;
+        double w2 = g[2] * g[2], s2 = w2 + 2.0 * nu * t;
+        double ex = x - g[0] - u * t, ey = y - g[1] - v * t;
+        acc += g[3] * (w2 / s2) * exp(-(ex * ex + ey * ey) / (2.0 * s2));
+    }
+    return acc;
+}
+
+__device__ double cosine_bell(const ExactArgs &e, double x, double y)
+{
+    double ex = x - e.p[0], ey = y - e.p[1];
+    double r = sqrt(ex * ex + ey * ey), R = e.p[2];
+    return r < R ? 0.5 * (1.0 + cos(M_PI * r / R)) : 0.0;
+}
+
+__device__ double point_value(const ExactArgs &e, double x, double y)
+{
+    int img = e.bc == BC_PER ? 1 : 0;
+    double acc = 0.0;
+    for (int ia = -img; ia <= img; ia++)
+        for (int ib = -img; ib <= img; ib++) {
+            double xx = x + ia * e.Lx, yy = y + ib * e.Ly;
+            double v;
+            if (e.kind == CCFV_IC_ROT_GAUSSIAN) v = rot_gauss(e, xx, yy);
+            else if (e.kind == CCFV_IC_GAUSSIANS) v = gaussians(e, xx, yy);
+            else v = cosine_bell(e, xx, yy);
+            acc += v;
+        }
+    return acc;
+}
+
+__global__ void exact_kernel(ExactArgs e, double *out)
+{
+    size_t total = (size_t)e.nx * e.ny;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        int i = (int)(idx / e.ny), j = (int)(idx % e.ny);
+        double x = e.x_lo + (i + 0.5) * e.dx, y = e.y_lo + (j + 0.5) * e.dy;
+        double xm = x - 0.5 * e.dx, xp = x + 0.5 * e.dx;
+        double ym = y - 0.5 * e.dy, yp = y + 0.5 * e.dy;
+        double v;
+        if (e.kind == CCFV_IC_SINSIN) {
+            const double tp = 2.0 * M_PI;
+            double u = e.p[0], w = e.p[1], nu = e.p[2], t = e.t;
+            double ax = (cos(tp * (xm - u * t)) - cos(tp * (xp - u * t))) / (tp * e.dx);
+            double ay = (cos(tp * (ym - w * t)) - cos(tp * (yp - w * t))) / (tp * e.dy);
+            v = ax * ay * exp(-2.0 * tp * tp * nu * t);
+        } else {
+            v = 1.0 / 36.0 *
+                (point_value(e, xm, ym) + point_value(e, xp, ym) +
+                 point_value(e, xm, yp) + point_value(e, xp, yp) +
+                 4.0 * (point_value(e, xm, y) + point_value(e, xp, y) +
+                        point_value(e, x, ym) + point_value(e, x, yp)) +
+                 16.0 * point_value(e, x, y));
+        }
+        out[idx] = v;
+    }
+}
+
+hipError_t launch_fill_exact(const ccfv_problem &p, const ccfv_ic &ic, double t,
+                             double *out, hipStream_t s)
+{
+    ExactArgs e;
+    e.nx = p.nx;
+    e.ny = p.ny;
+    e.bc = p.bc;
+    e.kind = ic.kind;
+    e.ng = ic.n;
+    e.x_lo = p.x_lo;
+    e.y_lo = p.y_lo;
+    e.dx = (p.x_hi - p.x_lo) / p.nx;
+    e.dy = (p.y_hi - p.y_lo) / p.ny;
+    e.Lx = p.x_hi - p.x_lo;
+    e.Ly = p.y_hi - p.y_lo;
+    e.nu = p.nu;
+    e.t = t;
+    for (int k = 0; k < 16; k++) e.p[k] = ic.p[k];
+    size_t total = (size_t)p.nx * p.ny;
+    int blocks = (int)((total + 255) / 256);
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(exact_kernel, dim3(blocks), dim3(256), 0, s, e, out);
+    return hipGetLastError();
+}
+
+}  // namespace ccfv

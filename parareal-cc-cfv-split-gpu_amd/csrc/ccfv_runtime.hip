@@ -1,0 +1,693 @@
+// ccfv_runtime.hip — host side of the C ABI (include/ccfv.h).
+//
+// A context owns one state field in HBM, the per-step-size solve tables and,
+// once Parareal runs, the slice arrays.  All work is queued on the context's
+// own stream (default priority).  Host code never touches field data except
+// through ccfv_set_state / ccfv_get_state.
+#include "ccfv_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+using namespace ccfv;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return fail(CCFV_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(expr)                                  \
+    do {                                              \
+        hipError_t e_ = (expr);                       \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+    } while (0)
+
+}  // namespace
+
+struct ccfv_ctx {
+    ccfv_problem P{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t ncell = 0;
+    double *state = nullptr;
+    int *d_flag = nullptr;   // crossed-feet / non-finite flag
+    int *h_flag = nullptr;   // pinned mirror
+    std::map<std::pair<int, double>, SolveTable> tables;
+    // tiles: [dir][0 = single field, 1 = batch]
+    Tile tile[2][2];
+    // Parareal buffers
+    int para_ns = 0;
+    double *U = nullptr, *G = nullptr, *F = nullptr, *Wk = nullptr;
+    double *partial = nullptr, *slot_sums = nullptr, *h_sums = nullptr;
+    int chain_blocks = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+Tile env_tile(const char *name, Tile dflt)
+{
+    const char *s = getenv(name);
+    if (!s) return dflt;
+    Tile t = dflt;
+    if (sscanf(s, "%d,%d", &t.w, &t.nw) != 2) return dflt;
+    return t;
+}
+
+bool tile_ok(Tile t, int n, int nl, Dir dir, bool post)
+{
+    if (t.w <= 0 || t.nw <= 0 || t.nw > 8 || t.w % t.nw || nl % t.w) return false;
+    if (dir == DIR_X && (t.w & 1)) return false;
+    return sweep_lds_bytes(n, t, post) <= 160 * 1024;
+}
+
+// default tiles; the in-repo autotuner overrides them per problem size
+Tile pick_tile(Dir dir, int n, int nl, bool batch, bool post)
+{
+    static const Tile cands_x[] = {{8, 4}, {4, 4}, {4, 2}, {2, 2}, {2, 1}};
+    static const Tile cands_y[] = {{4, 4}, {2, 2}, {1, 1}};
+    const Tile *c = dir == DIR_X ? cands_x : cands_y;
+    int nc = dir == DIR_X ? 5 : 3;
+    int start = 0;
+    if (dir == DIR_X && !batch) start = 1;  // more blocks for one field
+    for (int i = start; i < nc; i++)
+        if (tile_ok(c[i], n, nl, dir, post)) return c[i];
+    for (int i = 0; i < nc; i++)
+        if (tile_ok(c[i], n, nl, dir, post)) return c[i];
+    return Tile{0, 0};
+}
+
+double cell_width(const ccfv_problem &P, Dir dir)
+{
+    return dir == DIR_X ? (P.x_hi - P.x_lo) / P.nx : (P.y_hi - P.y_lo) / P.ny;
+}
+
+// Thomas factors of the constant tridiagonal matrix of one sweep direction
+// and step size (T2S4_Parareal.cu:550-561, :583-594, :666-673).  theta is
+// 1/4 for the x half-steps and 1/2 for the y step.
+int get_table(ccfv_ctx *c, Dir dir, double h, const SolveTable **out)
+{
+    auto key = std::make_pair((int)dir, h);
+    auto it = c->tables.find(key);
+    if (it != c->tables.end()) {
+        *out = &it->second;
+        return CCFV_OK;
+    }
+    const ccfv_problem &P = c->P;
+    const int n = dir == DIR_X ? P.nx : P.ny;
+    const double d = cell_width(P, dir);
+    const double theta = dir == DIR_X ? 0.25 : 0.5;
+    const double nu = P.nu;
+    const double r2 = h / (d * d);
+    SolveTable T;
+    T.ca = 1.0 / 12 - theta * nu * r2;
+    T.cb = 10.0 / 12 + 2.0 * theta * nu * r2;
+    T.coef = theta * nu * (h / d);
+    std::vector<double> diag(n, T.cb), p(n), den(n), mneg(n), pneg(n), z(n, 0.0);
+    const bool per = P.bc == CCFV_BC_PERIODIC;
+    if (per) {
+        // A = T + u v^T, gamma = -b, u = (gamma,0..0,a), v = (1,0..0,a/gamma)
+        double gamma = -T.cb;
+        T.vl = T.ca / gamma;
+        diag[0] = T.cb - gamma;
+        diag[n - 1] = T.cb - T.ca * T.vl;
+    }
+    T.diag0 = diag[0];
+    p[0] = T.ca / diag[0];
+    den[0] = 1.0 / diag[0];
+    for (int i = 1; i < n; i++) {
+        den[i] = 1.0 / (diag[i] - p[i - 1] * T.ca);
+        p[i] = T.ca * den[i];
+    }
+    mneg[0] = 0.0;
+    for (int i = 1; i < n; i++) mneg[i] = -T.ca * den[i];
+    for (int i = 0; i < n; i++) pneg[i] = -p[i];
+    pneg[n - 1] = 0.0;
+    if (per) {
+        // z = T^{-1} u by the same recurrences
+        std::vector<double> u(n, 0.0), q(n);
+        u[0] = -T.cb;
+        u[n - 1] = T.ca;
+        q[0] = u[0] / diag[0];
+        for (int i = 1; i < n; i++) q[i] = (u[i] - q[i - 1] * T.ca) * den[i];
+        z[n - 1] = q[n - 1];
+        for (int i = n - 2; i >= 0; i--) z[i] = q[i] - p[i] * z[i + 1];
+        T.smden = 1.0 + (z[0] + T.vl * z[n - 1]);
+    }
+    std::vector<double> tab(4 * (size_t)n);
+    std::copy(mneg.begin(), mneg.end(), tab.begin());
+    std::copy(den.begin(), den.end(), tab.begin() + n);
+    std::copy(pneg.begin(), pneg.end(), tab.begin() + 2 * n);
+    std::copy(z.begin(), z.end(), tab.begin() + 3 * n);
+    HIPCHK(hipMalloc(&T.dev, sizeof(double) * tab.size()));
+    HIPCHK(hipMemcpy(T.dev, tab.data(), sizeof(double) * tab.size(),
+                     hipMemcpyHostToDevice));
+    auto res = c->tables.emplace(key, T);
+    *out = &res.first->second;
+    return CCFV_OK;
+}
+
+SweepArgs make_args(ccfv_ctx *c, Dir dir, int substep, const SolveTable &T,
+                    double h, double t0, double tDT, int field0, double jdt)
+{
+    const ccfv_problem &P = c->P;
+    SweepArgs a;
+    memset(&a, 0, sizeof(a));
+    a.ld = P.ny;
+    if (dir == DIR_X) {
+        a.n = P.nx;
+        a.nl = P.ny;
+        a.lo = P.x_lo;
+        a.hi = P.x_hi;
+        a.olo = P.y_lo;
+        a.eps = 1e-15;   // :323-324
+        a.tau = 0.5 * h; // the x sub-steps trace over dt/2 (:215-217)
+    } else {
+        a.n = P.ny;
+        a.nl = P.nx;
+        a.lo = P.y_lo;
+        a.hi = P.y_hi;
+        a.olo = P.x_lo;
+        a.eps = 1e-16;   // :414-415
+        a.tau = h;       // the y sub-step traces over dt (:219-221)
+    }
+    a.d = cell_width(P, dir);
+    a.od = cell_width(P, dir == DIR_X ? DIR_Y : DIR_X);
+    a.quirk = (P.flags & CCFV_FLAG_DERIV_GHOST_QUIRK) ? 1 : 0;
+    a.coef = T.coef;
+    a.ca = T.ca;
+    a.diag0 = T.diag0;
+    a.vl = T.vl;
+    a.smden = T.smden;
+    a.tab = T.dev;
+    a.vel_kind = P.vel_kind;
+    a.v0 = P.vel[0];
+    a.v1 = P.vel[1];
+    a.v2 = P.vel[2];
+    a.t0 = t0;
+    a.tDT = tDT;
+    a.jdt = jdt;
+    a.h = h;
+    a.field0 = field0;
+    a.substep = substep;
+    a.flag = c->d_flag;
+    return a;
+}
+
+bool post_flag(const ccfv_ctx *c)
+{
+    return (c->P.flags & CCFV_FLAG_STEP3_POSTREMAP) != 0;
+}
+
+// One Strang step x(h/2) -> y(h) -> x(h/2) on nf fields (GpuOneStep,
+// T2S4_Parareal.cu:785-968).  The first sweep reads `in` and writes `out`;
+// the other two work in place on `out` (every sweep is line-local).
+int step_fields(ccfv_ctx *c, const double *in, double *out, int nf, double t0,
+                double tDT, int field0, double jdt, double h, Epi epi = EPI_NONE,
+                double *g_cache = nullptr, const double *f_in = nullptr,
+                double *u_next = nullptr, double *partial = nullptr)
+{
+    const SolveTable *tx, *ty;
+    int rc = get_table(c, DIR_X, h, &tx);
+    if (rc) return rc;
+    rc = get_table(c, DIR_Y, h, &ty);
+    if (rc) return rc;
+    const bool batch = nf > 1;
+    const bool post = post_flag(c);
+    const long long fs = (long long)c->ncell;
+    SweepArgs a = make_args(c, DIR_X, 1, *tx, h, t0, tDT, field0, jdt);
+    a.in = in;
+    a.out = out;
+    a.fstride = fs;
+    HIPCHK(launch_sweep(DIR_X, c->P.bc, false, EPI_NONE, a, nf,
+                        c->tile[DIR_X][batch], c->stream));
+    SweepArgs b = make_args(c, DIR_Y, 2, *ty, h, t0, tDT, field0, jdt);
+    b.in = out;
+    b.out = out;
+    b.fstride = fs;
+    HIPCHK(launch_sweep(DIR_Y, c->P.bc, false, EPI_NONE, b, nf,
+                        c->tile[DIR_Y][batch], c->stream));
+    SweepArgs x3 = make_args(c, DIR_X, 3, *tx, h, t0, tDT, field0, jdt);
+    x3.in = out;
+    x3.out = out;
+    x3.fstride = fs;
+    x3.g_cache = g_cache;
+    x3.f_in = f_in;
+    x3.u_next = u_next;
+    x3.partial = partial;
+    HIPCHK(launch_sweep(DIR_X, c->P.bc, post, epi, x3, nf,
+                        c->tile[DIR_X][batch], c->stream));
+    return CCFV_OK;
+}
+
+int check_flag(ccfv_ctx *c)
+{
+    HIPCHK(hipMemcpyAsync(c->h_flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (*c->h_flag) {
+        *c->h_flag = 0;
+        HIPCHK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return fail(CCFV_ECFL,
+                    "characteristic feet crossed: the time step is too large "
+                    "for this velocity field");
+    }
+    return CCFV_OK;
+}
+
+int validate_problem(const ccfv_problem *p)
+{
+    if (!p) return fail(CCFV_EINVAL, "problem is NULL");
+    if (!supported_n(p->nx) || !supported_n(p->ny))
+        return fail(CCFV_EINVAL,
+                    "unsupported grid: nx and ny must be even, 8..2048, and a "
+                    "multiple of ceil-pow2(n/64)");
+    if (p->nx < 8 || p->ny < 8) return fail(CCFV_EINVAL, "grid too small");
+    if (p->bc != CCFV_BC_REFERENCE && p->bc != CCFV_BC_PERIODIC)
+        return fail(CCFV_EINVAL, "unknown boundary mode");
+    if (p->vel_kind < 0 || p->vel_kind > CCFV_VEL_SWIRL)
+        return fail(CCFV_EINVAL, "unknown velocity kind");
+    if (!(p->x_hi > p->x_lo) || !(p->y_hi > p->y_lo))
+        return fail(CCFV_EINVAL, "empty box");
+    if (!(p->nu >= 0.0) || !std::isfinite(p->nu))
+        return fail(CCFV_EINVAL, "diffusion coefficient must be >= 0");
+    if (p->vel_kind == CCFV_VEL_SWIRL && !(p->vel[0] > 0.0))
+        return fail(CCFV_EINVAL, "swirl period must be > 0");
+    if (p->bc == CCFV_BC_PERIODIC && (p->flags & CCFV_FLAG_STEP3_POSTREMAP))
+        return fail(CCFV_EINVAL,
+                    "the third-sub-step post-remap derivative is a reference "
+                    "(closed box) behaviour only");
+    return CCFV_OK;
+}
+
+void free_para(ccfv_ctx *c)
+{
+    hipFree(c->U);
+    hipFree(c->G);
+    hipFree(c->F);
+    hipFree(c->Wk);
+    hipFree(c->partial);
+    hipFree(c->slot_sums);
+    if (c->h_sums) hipHostFree(c->h_sums);
+    c->U = c->G = c->F = c->Wk = c->partial = c->slot_sums = c->h_sums = nullptr;
+    c->para_ns = 0;
+}
+
+int ensure_para(ccfv_ctx *c, int ns)
+{
+    if (c->para_ns == ns) return CCFV_OK;
+    free_para(c);
+    const size_t fb = sizeof(double) * c->ncell;
+    c->chain_blocks = c->P.ny / c->tile[DIR_X][0].w;
+    hipError_t e;
+    if ((e = hipMalloc(&c->U, fb * (ns + 1))) != hipSuccess ||
+        (e = hipMalloc(&c->G, fb * ns)) != hipSuccess ||
+        (e = hipMalloc(&c->F, fb * ns)) != hipSuccess ||
+        (e = hipMalloc(&c->Wk, fb)) != hipSuccess ||
+        (e = hipMalloc(&c->partial, sizeof(double) * 2 * c->chain_blocks *
+                                        (size_t)(ns + 1))) != hipSuccess ||
+        (e = hipMalloc(&c->slot_sums, sizeof(double) * 2 * (ns + 1))) !=
+            hipSuccess ||
+        (e = hipHostMalloc(&c->h_sums, sizeof(double) * 2 * (ns + 1))) !=
+            hipSuccess) {
+        free_para(c);
+        return fail(CCFV_ENOMEM, std::string("Parareal buffers: ") +
+                                     hipGetErrorString(e));
+    }
+    c->para_ns = ns;
+    return CCFV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ccfv_last_error(void) { return g_err.c_str(); }
+
+int ccfv_version(int32_t *major, int32_t *minor)
+{
+    if (major) *major = 0;
+    if (minor) *minor = 1;
+    return CCFV_OK;
+}
+
+int ccfv_device_count(int32_t *count)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        if (count) *count = 0;
+        return hip_fail(e, "hipGetDeviceCount");
+    }
+    if (count) *count = n;
+    return CCFV_OK;
+}
+
+int ccfv_problem_reference(ccfv_problem *p, int32_t n, double K)
+{
+    if (!p) return fail(CCFV_EINVAL, "NULL problem");
+    memset(p, 0, sizeof(*p));
+    p->nx = p->ny = n;                    // :1037-1039
+    p->bc = CCFV_BC_REFERENCE;
+    p->vel_kind = CCFV_VEL_SOLID_BODY;    // Vx=-4y, Vy=4x (:44-45)
+    p->vel[0] = 4.0;
+    p->x_lo = -1.0;                       // :30-33
+    p->x_hi = 1.0;
+    p->y_lo = -1.0;
+    p->y_hi = 1.0;
+    p->nu = K;                            // :1047-1048
+    p->flags = CCFV_FLAGS_REFERENCE;
+    return CCFV_OK;
+}
+
+int ccfv_ic_reference(ccfv_ic *ic, double K)
+{
+    (void)K;  // the diffusion enters through the problem's nu
+    if (!ic) return fail(CCFV_EINVAL, "NULL ic");
+    memset(ic, 0, sizeof(*ic));
+    ic->kind = CCFV_IC_ROT_GAUSSIAN;      // C_Exact (:51-56), x0,y0,sigma :27-29
+    ic->p[0] = -0.35;
+    ic->p[1] = 0.0;
+    ic->p[2] = 0.005;
+    ic->p[3] = 4.0;
+    return CCFV_OK;
+}
+
+int ccfv_create(const ccfv_problem *p, ccfv_ctx **out)
+{
+    if (!out) return fail(CCFV_EINVAL, "NULL out");
+    *out = nullptr;
+    int rc = validate_problem(p);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(CCFV_EDEVICE, "no HIP device available");
+    ccfv_ctx *c = new ccfv_ctx();
+    c->P = *p;
+    HIPCHK(hipGetDevice(&c->device));
+    c->ncell = (size_t)p->nx * p->ny;
+    const bool post = (p->flags & CCFV_FLAG_STEP3_POSTREMAP) != 0;
+    for (int b = 0; b < 2; b++) {
+        c->tile[DIR_X][b] = pick_tile(DIR_X, p->nx, p->ny, b, post);
+        c->tile[DIR_Y][b] = pick_tile(DIR_Y, p->ny, p->nx, b, false);
+    }
+    c->tile[DIR_X][0] = env_tile("CCFV_TILE_X1", c->tile[DIR_X][0]);
+    c->tile[DIR_X][1] = env_tile("CCFV_TILE_XB", c->tile[DIR_X][1]);
+    c->tile[DIR_Y][0] = env_tile("CCFV_TILE_Y1", c->tile[DIR_Y][0]);
+    c->tile[DIR_Y][1] = env_tile("CCFV_TILE_YB", c->tile[DIR_Y][1]);
+    for (int d = 0; d < 2; d++)
+        for (int b = 0; b < 2; b++)
+            if (!tile_ok(c->tile[d][b], d == DIR_X ? p->nx : p->ny,
+                         d == DIR_X ? p->ny : p->nx, (Dir)d, post && d == DIR_X)) {
+                delete c;
+                return fail(CCFV_EINVAL, "no valid tile for this grid");
+            }
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) !=
+            hipSuccess ||
+        (e = hipMalloc(&c->state, sizeof(double) * c->ncell)) != hipSuccess ||
+        (e = hipMalloc(&c->d_flag, sizeof(int))) != hipSuccess ||
+        (e = hipHostMalloc(&c->h_flag, sizeof(int))) != hipSuccess ||
+        (e = hipEventCreate(&c->ev0)) != hipSuccess ||
+        (e = hipEventCreate(&c->ev1)) != hipSuccess) {
+        ccfv_destroy(c);
+        return fail(CCFV_ENOMEM, std::string("context allocation: ") +
+                                     hipGetErrorString(e));
+    }
+    *c->h_flag = 0;
+    hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream);
+    hipMemsetAsync(c->state, 0, sizeof(double) * c->ncell, c->stream);
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) {
+        ccfv_destroy(c);
+        return hip_fail(e, "context init");
+    }
+    *out = c;
+    return CCFV_OK;
+}
+
+void ccfv_destroy(ccfv_ctx *c)
+{
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    free_para(c);
+    for (auto &kv : c->tables) hipFree(kv.second.dev);
+    hipFree(c->state);
+    hipFree(c->d_flag);
+    if (c->h_flag) hipHostFree(c->h_flag);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int ccfv_set_state(ccfv_ctx *c, const double *host)
+{
+    if (!c || !host) return fail(CCFV_EINVAL, "NULL argument");
+    HIPCHK(hipMemcpyAsync(c->state, host, sizeof(double) * c->ncell,
+                          hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CCFV_OK;
+}
+
+int ccfv_get_state(ccfv_ctx *c, double *host)
+{
+    if (!c || !host) return fail(CCFV_EINVAL, "NULL argument");
+    HIPCHK(hipMemcpyAsync(host, c->state, sizeof(double) * c->ncell,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CCFV_OK;
+}
+
+int ccfv_set_state_device(ccfv_ctx *c, const double *dev)
+{
+    if (!c || !dev) return fail(CCFV_EINVAL, "NULL argument");
+    HIPCHK(launch_copy(dev, c->state, c->ncell, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CCFV_OK;
+}
+
+int ccfv_get_state_device(ccfv_ctx *c, double *dev)
+{
+    if (!c || !dev) return fail(CCFV_EINVAL, "NULL argument");
+    HIPCHK(launch_copy(c->state, dev, c->ncell, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CCFV_OK;
+}
+
+int ccfv_fill_exact(ccfv_ctx *c, const ccfv_ic *ic, double t)
+{
+    if (!c || !ic) return fail(CCFV_EINVAL, "NULL argument");
+    if (ic->kind < 0 || ic->kind > CCFV_IC_COSINE_BELL)
+        return fail(CCFV_EINVAL, "unknown initial condition");
+    HIPCHK(launch_fill_exact(c->P, *ic, t, c->state, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CCFV_OK;
+}
+
+int ccfv_exact_to_host(ccfv_ctx *c, const ccfv_ic *ic, double t, double *host)
+{
+    if (!c || !ic || !host) return fail(CCFV_EINVAL, "NULL argument");
+    if (ic->kind < 0 || ic->kind > CCFV_IC_COSINE_BELL)
+        return fail(CCFV_EINVAL, "unknown initial condition");
+    double *tmp = nullptr;
+    HIPCHK(hipMalloc(&tmp, sizeof(double) * c->ncell));
+    hipError_t e = launch_fill_exact(c->P, *ic, t, tmp, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(host, tmp, sizeof(double) * c->ncell,
+                           hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(tmp);
+    if (e != hipSuccess) return hip_fail(e, "exact_to_host");
+    return CCFV_OK;
+}
+
+int ccfv_step(ccfv_ctx *c, double tn, double h)
+{
+    if (!c) return fail(CCFV_EINVAL, "NULL context");
+    if (!(h > 0.0) || !std::isfinite(h) || !std::isfinite(tn))
+        return fail(CCFV_EINVAL, "step size must be finite and > 0");
+    int rc = step_fields(c, c->state, c->state, 1, tn, 0.0, 0, 0.0, h);
+    if (rc) return rc;
+    return check_flag(c);
+}
+
+int ccfv_coarse(ccfv_ctx *c, double tn, double DT)
+{
+    // CoarsePropagator (T2S4_Parareal.cu:971-974): one step of DT
+    return ccfv_step(c, tn, DT);
+}
+
+int ccfv_fine(ccfv_ctx *c, double tn, double DT, int32_t J)
+{
+    // FinePropagator (T2S4_Parareal.cu:977-984): J steps of DT/J
+    if (!c) return fail(CCFV_EINVAL, "NULL context");
+    if (J < 1) return fail(CCFV_EINVAL, "J must be >= 1");
+    if (!(DT > 0.0) || !std::isfinite(DT))
+        return fail(CCFV_EINVAL, "DT must be finite and > 0");
+    double dt = DT / (double)J;
+    for (int j = 0; j < J; j++) {
+        int rc = step_fields(c, c->state, c->state, 1, tn, 0.0, 0, j * dt, dt);
+        if (rc) return rc;
+    }
+    return check_flag(c);
+}
+
+int ccfv_plain_solve(ccfv_ctx *c, double t0, double t1, int32_t nt)
+{
+    // serial fine loop (T2S4_Parareal.cu:1060, :1234-1236)
+    if (!c) return fail(CCFV_EINVAL, "NULL context");
+    if (nt < 1) return fail(CCFV_EINVAL, "nt must be >= 1");
+    if (!(t1 > t0) || !std::isfinite(t0) || !std::isfinite(t1))
+        return fail(CCFV_EINVAL, "need finite t1 > t0");
+    double dt = (t1 - t0) / nt;
+    for (int n = 0; n < nt; n++) {
+        int rc = step_fields(c, c->state, c->state, 1, t0 + n * dt, 0.0, 0, 0.0, dt);
+        if (rc) return rc;
+    }
+    return check_flag(c);
+}
+
+int ccfv_parareal_correction(ccfv_ctx *c, const double *g_new,
+                             const double *g_old, const double *f, double *u,
+                             size_t n)
+{
+    if (!c || !g_new || !g_old || !f || !u) return fail(CCFV_EINVAL, "NULL argument");
+    int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+    if (blocks < 1) blocks = 1;
+    HIPCHK(launch_correction(g_new, g_old, f, u, n, nullptr, blocks, c->stream));
+    return CCFV_OK;
+}
+
+int ccfv_sync(ccfv_ctx *c)
+{
+    if (!c) return fail(CCFV_EINVAL, "NULL context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CCFV_OK;
+}
+
+int ccfv_parareal_solve(ccfv_ctx *c, double t0, double t1, int32_t nt_fine,
+                        int32_t ns, int32_t kmax, double tol,
+                        ccfv_parareal_report *rep)
+{
+    if (!c) return fail(CCFV_EINVAL, "NULL context");
+    if (ns < 1 || nt_fine < ns || nt_fine % ns)
+        return fail(CCFV_EINVAL, "nt_fine must be a positive multiple of n_slices");
+    if (kmax < 0 || kmax > CCFV_MAX_ITERS)
+        return fail(CCFV_EINVAL, "kmax must be in [0, 64]");
+    if (!(t1 > t0)) return fail(CCFV_EINVAL, "need t1 > t0");
+    int rc = ensure_para(c, ns);
+    if (rc) return rc;
+    const size_t N = c->ncell;
+    const double DT = (t1 - t0) / ns;   // :1251
+    const int J = nt_fine / ns;         // :1252
+    const double dt = DT / (double)J;   // :979
+    const int nb = c->chain_blocks;
+    // tables up front (host-side work stays out of the timed loop)
+    const SolveTable *tt;
+    if ((rc = get_table(c, DIR_X, DT, &tt)) || (rc = get_table(c, DIR_Y, DT, &tt)) ||
+        (rc = get_table(c, DIR_X, dt, &tt)) || (rc = get_table(c, DIR_Y, dt, &tt)))
+        return rc;
+    ccfv_parareal_report R;
+    memset(&R, 0, sizeof(R));
+    R.n_slices = ns;
+    R.j_fine = J;
+    std::vector<double> slot_u2(ns + 1, 0.0);  // |U[s]|^2 of the current iterate
+
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    HIPCHK(launch_copy(c->state, c->U, N, c->stream));
+    // k = 0: U[n+1] = G(U[n]), caching G (T2S4_Parareal.cu:1264-1280)
+    for (int n = 0; n < ns; n++) {
+        rc = step_fields(c, c->U + n * N, c->Wk, 1, t0, DT, n, 0.0, DT, EPI_INIT,
+                         c->G + n * N, nullptr, c->U + (n + 1) * N, nullptr);
+        if (rc) return rc;
+    }
+    R.coarse_steps_run = ns;
+    int k = 0;
+    bool converged = false;
+    for (k = 0; k < kmax; k++) {
+        // fine propagators of the slices that can still change, batched:
+        // F[n] = F(U[n]), n = k..ns-1 (T2S4_Parareal.cu:1319-1324)
+        const int nf = ns - k;
+        for (int j = 0; j < J; j++) {
+            const double *src = j == 0 ? c->U + k * N : c->F + k * N;
+            rc = step_fields(c, src, c->F + k * N, nf, t0, DT, k, j * dt, dt);
+            if (rc) return rc;
+        }
+        R.fine_steps_run += (double)nf * J;
+        // slot k+1: U[k] is unchanged, so G(U^{k+1}_k) == G cache (bitwise)
+        HIPCHK(launch_correction(c->G + k * N, c->G + k * N, c->F + k * N,
+                                 c->U + (k + 1) * N, N,
+                                 c->partial + 2 * (size_t)nb * (k + 1), nb,
+                                 c->stream));
+        // coarse chain with the correction fused into the last sweep
+        // (T2S4_Parareal.cu:1302-1330)
+        for (int n = k + 1; n < ns; n++) {
+            rc = step_fields(c, c->U + n * N, c->Wk, 1, t0, DT, n, 0.0, DT,
+                             EPI_CORRECT, c->G + n * N, c->F + n * N,
+                             c->U + (n + 1) * N,
+                             c->partial + 2 * (size_t)nb * (n + 1));
+            if (rc) return rc;
+        }
+        R.coarse_steps_run += ns - k - 1;
+        HIPCHK(launch_reduce_segments(c->partial, nb, k + 1, ns - k, c->slot_sums,
+                                      c->stream));
+        HIPCHK(hipMemcpyAsync(c->h_sums, c->slot_sums,
+                              sizeof(double) * 2 * (ns + 1),
+                              hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        double num = 0.0, den = 0.0;
+        for (int s = 1; s <= ns; s++) {
+            if (s >= k + 1) {
+                num += c->h_sums[2 * s];
+                slot_u2[s] = c->h_sums[2 * s + 1];
+            }
+            den += slot_u2[s];
+        }
+        double inc = den > 0 ? std::sqrt(num / den) : std::sqrt(num);
+        R.increment[k] = inc;
+        if (!std::isfinite(inc)) {
+            k++;
+            R.iterations = k;
+            if (rep) *rep = R;
+            return fail(CCFV_ENONFINITE, "Parareal iterate is not finite");
+        }
+        if (tol > 0 && inc <= tol) {
+            converged = true;
+            k++;
+            break;
+        }
+    }
+    R.iterations = k;
+    R.converged = converged ? 1 : 0;
+    HIPCHK(launch_copy(c->U + (size_t)ns * N, c->state, N, c->stream));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    rc = check_flag(c);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    R.wall_ms = ms;
+    if (rep) *rep = R;
+    if (rc) return rc;
+    if (tol > 0 && !converged)
+        return fail(CCFV_ENOTCONV, "Parareal did not reach the tolerance within kmax");
+    return CCFV_OK;
+}
+
+}  // extern "C"
