@@ -197,6 +197,29 @@ int ccfv_parareal_correction(ccfv_ctx *ctx, const double *g_new,
 /* Block until all work queued on the context's stream is done. */
 int ccfv_sync(ccfv_ctx *ctx);
 
+/* ---- kernel timing (instrumentation; no counterpart in the reference,
+ * which only reads clock() around the whole program, :1137, :1357) -------
+ * While enabled, every sweep launch on the context's stream is bracketed by
+ * hipEvents and accumulated per kernel class together with its algorithmic
+ * HBM bytes (8 B read + 8 B written per cell and field; the fused Parareal
+ * epilogue adds 32 B per cell).  Enabling clears the totals.  Meant for a
+ * dedicated measurement run: the events add launch gaps. */
+#define CCFV_KSTAT_X_BATCH   0   /* x sweeps over a batch of fields           */
+#define CCFV_KSTAT_Y_BATCH   1   /* y sweeps over a batch of fields           */
+#define CCFV_KSTAT_X_SINGLE  2   /* x sweeps of one field                     */
+#define CCFV_KSTAT_Y_SINGLE  3   /* y sweeps of one field                     */
+#define CCFV_KSTAT_X_EPILOG  4   /* last x sweep of a coarse step + correction */
+#define CCFV_KSTAT_COUNT     5
+typedef struct ccfv_kernel_stat {
+    int32_t kind;
+    int32_t launches;
+    double  total_ms;   /* summed hipEvent durations                      */
+    double  bytes;      /* summed algorithmic HBM bytes                   */
+} ccfv_kernel_stat;
+int ccfv_profile_enable(ccfv_ctx *ctx, int32_t enable);
+/* Fill out[CCFV_KSTAT_COUNT]; synchronises the stream. */
+int ccfv_profile_read(ccfv_ctx *ctx, ccfv_kernel_stat *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,9 @@ _SIGS = {
     "ccfv_parareal_correction": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4
                                  + [ctypes.c_size_t]),
     "ccfv_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "ccfv_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "ccfv_profile_read": (ctypes.c_int, [ctypes.c_void_p,
+                                         ctypes.POINTER(types.KernelStat)]),
 }
 
 _lib = None
@@ -195,6 +198,16 @@ class Context:
 
     def sync(self):
         _check(load().ccfv_sync(self._h), "ccfv_sync")
+
+    def profile_enable(self, on: bool = True):
+        _check(load().ccfv_profile_enable(self._h, int(on)), "ccfv_profile_enable")
+
+    def profile_read(self) -> dict:
+        arr = (types.KernelStat * len(types.KSTAT_NAMES))()
+        _check(load().ccfv_profile_read(self._h, arr), "ccfv_profile_read")
+        return {types.KSTAT_NAMES[s.kind]: {"launches": s.launches,
+                                            "total_ms": s.total_ms,
+                                            "bytes": s.bytes} for s in arr}
 
 
 __all__ = ["Context", "CcfvError", "load", "device_count", "types", "Problem",

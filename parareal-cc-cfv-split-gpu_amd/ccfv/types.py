@@ -109,6 +109,15 @@ class ParaReport(ctypes.Structure):
         return [self.increment[i] for i in range(self.iterations)]
 
 
+KSTAT_NAMES = ["x_batch", "y_batch", "x_single", "y_single", "x_epilog"]
+
+
+class KernelStat(ctypes.Structure):
+    """ccfv_kernel_stat."""
+    _fields_ = [("kind", ctypes.c_int32), ("launches", ctypes.c_int32),
+                ("total_ms", ctypes.c_double), ("bytes", ctypes.c_double)]
+
+
 def make_problem(nx, ny, *, bc, vel_kind, vel, nu, box=(0.0, 1.0, 0.0, 1.0),
                  flags=0) -> Problem:
     p = Problem()

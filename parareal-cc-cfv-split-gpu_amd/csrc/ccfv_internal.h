@@ -27,6 +27,7 @@ struct SolveTable {
     double diag0 = 0;       // first diagonal entry of the factored matrix
     double coef = 0;        // explicit diffusion factor (theta*nu)*(h/d)
     double vl = 0, smden = 1;  // Sherman-Morrison (periodic)
+    int sf = 6, sb = 6;        // lane-scan doubling steps (forward, backward)
 };
 
 // Everything one sweep kernel needs, passed by value.
@@ -38,6 +39,9 @@ struct SweepArgs {
     int nl;             // lines (cells in the other direction)
     int ld;             // row length of the field (ny)
     int quirk;          // reference ghost selection in the derivative stencil
+    int uniform;        // along-line velocity independent of the along-line
+                        // coordinate (uniform foot offset on a line)
+    int sf, sb;         // lane-scan doubling steps, forward / backward
     double lo, hi, d;   // along-line box and cell width
     double olo, od;     // other-direction box start and cell width
     double eps;         // remap clip epsilon (closed box)

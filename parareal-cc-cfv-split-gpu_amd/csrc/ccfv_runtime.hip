@@ -58,6 +58,16 @@ struct ccfv_ctx {
     double *partial = nullptr, *slot_sums = nullptr, *h_sums = nullptr;
     int chain_blocks = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // kernel timing
+    bool prof = false;
+    struct Rec {
+        int kind;
+        double bytes;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> ev_pool;
+    ccfv_kernel_stat stats[CCFV_KSTAT_COUNT];
 };
 
 namespace {
@@ -151,6 +161,35 @@ int get_table(ccfv_ctx *c, Dir dir, double h, const SolveTable **out)
         for (int i = n - 2; i >= 0; i--) z[i] = q[i] - p[i] * z[i + 1];
         T.smden = 1.0 + (z[0] + T.vl * z[n - 1]);
     }
+    // Doubling steps of the lane scans: after s steps a lane's carry neglects
+    // the product of the factors over 2^s whole chunks; stop once that is
+    // below 1e-18 (far under FP64 rounding of the carried value).
+    {
+        const int ch = chunk_for(n), nl = n / ch;
+        double pf = 0.0, pb = 0.0;
+        for (int l = 0; l < nl; l++) {
+            double f = 1.0, b = 1.0;
+            for (int k = 0; k < ch; k++) {
+                f *= std::fabs(mneg[l * ch + k]);
+                b *= std::fabs(pneg[l * ch + k]);
+            }
+            pf = std::max(pf, f);
+            pb = std::max(pb, b);
+        }
+        auto steps = [nl](double p) {
+            int full = 0;
+            while ((1 << full) < nl) full++;
+            if (full < 1) full = 1;
+            if (p <= 0.0) return 1;
+            if (p >= 1.0) return full;
+            double need = std::log(1e-18) / std::log(p);  // chunks to span
+            int s = 1;
+            while ((double)(1 << s) < need && s < full) s++;
+            return std::min(s, full);
+        };
+        T.sf = steps(pf);
+        T.sb = steps(pb);
+    }
     std::vector<double> tab(4 * (size_t)n);
     std::copy(mneg.begin(), mneg.end(), tab.begin());
     std::copy(den.begin(), den.end(), tab.begin() + n);
@@ -191,6 +230,10 @@ SweepArgs make_args(ccfv_ctx *c, Dir dir, int substep, const SolveTable &T,
     a.d = cell_width(P, dir);
     a.od = cell_width(P, dir == DIR_X ? DIR_Y : DIR_X);
     a.quirk = (P.flags & CCFV_FLAG_DERIV_GHOST_QUIRK) ? 1 : 0;
+    a.uniform = P.bc == CCFV_BC_PERIODIC && (P.vel_kind == CCFV_VEL_SOLID_BODY ||
+                                             P.vel_kind == CCFV_VEL_CONSTANT);
+    a.sf = T.sf;
+    a.sb = T.sb;
     a.coef = T.coef;
     a.ca = T.ca;
     a.diag0 = T.diag0;
@@ -209,6 +252,54 @@ SweepArgs make_args(ccfv_ctx *c, Dir dir, int substep, const SolveTable &T,
     a.substep = substep;
     a.flag = c->d_flag;
     return a;
+}
+
+hipEvent_t pool_event(ccfv_ctx *c)
+{
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// fold finished records into the totals (caller has synchronised)
+void drain_records(ccfv_ctx *c)
+{
+    for (auto &r : c->recs) {
+        float ms = 0.f;
+        if (r.a && r.b && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            ccfv_kernel_stat &s = c->stats[r.kind];
+            s.launches += 1;
+            s.total_ms += ms;
+            s.bytes += r.bytes;
+        }
+        if (r.a) c->ev_pool.push_back(r.a);
+        if (r.b) c->ev_pool.push_back(r.b);
+    }
+    c->recs.clear();
+}
+
+// launch one sweep, timed when profiling is on
+hipError_t timed_sweep(ccfv_ctx *c, int kind, double bytes, Dir dir, bool post,
+                       Epi epi, const SweepArgs &a, int nf, Tile t)
+{
+    if (!c->prof)
+        return launch_sweep(dir, c->P.bc, post, epi, a, nf, t, c->stream);
+    if (c->recs.size() > 4096) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return e;
+        drain_records(c);
+    }
+    ccfv_ctx::Rec r{kind, bytes, pool_event(c), pool_event(c)};
+    if (r.a) (void)hipEventRecord(r.a, c->stream);
+    hipError_t e = launch_sweep(dir, c->P.bc, post, epi, a, nf, t, c->stream);
+    if (r.b) (void)hipEventRecord(r.b, c->stream);
+    c->recs.push_back(r);
+    return e;
 }
 
 bool post_flag(const ccfv_ctx *c)
@@ -236,14 +327,15 @@ int step_fields(ccfv_ctx *c, const double *in, double *out, int nf, double t0,
     a.in = in;
     a.out = out;
     a.fstride = fs;
-    HIPCHK(launch_sweep(DIR_X, c->P.bc, false, EPI_NONE, a, nf,
-                        c->tile[DIR_X][batch], c->stream));
+    const double fbytes = 16.0 * (double)c->ncell * nf;
+    HIPCHK(timed_sweep(c, batch ? CCFV_KSTAT_X_BATCH : CCFV_KSTAT_X_SINGLE, fbytes,
+                       DIR_X, false, EPI_NONE, a, nf, c->tile[DIR_X][batch]));
     SweepArgs b = make_args(c, DIR_Y, 2, *ty, h, t0, tDT, field0, jdt);
     b.in = out;
     b.out = out;
     b.fstride = fs;
-    HIPCHK(launch_sweep(DIR_Y, c->P.bc, false, EPI_NONE, b, nf,
-                        c->tile[DIR_Y][batch], c->stream));
+    HIPCHK(timed_sweep(c, batch ? CCFV_KSTAT_Y_BATCH : CCFV_KSTAT_Y_SINGLE, fbytes,
+                       DIR_Y, false, EPI_NONE, b, nf, c->tile[DIR_Y][batch]));
     SweepArgs x3 = make_args(c, DIR_X, 3, *tx, h, t0, tDT, field0, jdt);
     x3.in = out;
     x3.out = out;
@@ -252,8 +344,13 @@ int step_fields(ccfv_ctx *c, const double *in, double *out, int nf, double t0,
     x3.f_in = f_in;
     x3.u_next = u_next;
     x3.partial = partial;
-    HIPCHK(launch_sweep(DIR_X, c->P.bc, post, epi, x3, nf,
-                        c->tile[DIR_X][batch], c->stream));
+    int kind = epi != EPI_NONE ? CCFV_KSTAT_X_EPILOG
+                               : (batch ? CCFV_KSTAT_X_BATCH : CCFV_KSTAT_X_SINGLE);
+    double xbytes = epi == EPI_CORRECT ? 48.0 * (double)c->ncell
+                    : epi == EPI_INIT  ? 24.0 * (double)c->ncell
+                                       : fbytes;
+    HIPCHK(timed_sweep(c, kind, xbytes, DIR_X, post, epi, x3, nf,
+                       c->tile[DIR_X][batch]));
     return CCFV_OK;
 }
 
@@ -448,7 +545,9 @@ void ccfv_destroy(ccfv_ctx *c)
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
     free_para(c);
-    for (auto &kv : c->tables) hipFree(kv.second.dev);
+    drain_records(c);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    for (auto &kv : c->tables) (void)hipFree(kv.second.dev);
     hipFree(c->state);
     hipFree(c->d_flag);
     if (c->h_flag) hipHostFree(c->h_flag);
@@ -573,6 +672,29 @@ int ccfv_parareal_correction(ccfv_ctx *c, const double *g_new,
     int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
     if (blocks < 1) blocks = 1;
     HIPCHK(launch_correction(g_new, g_old, f, u, n, nullptr, blocks, c->stream));
+    return CCFV_OK;
+}
+
+int ccfv_profile_enable(ccfv_ctx *c, int32_t enable)
+{
+    if (!c) return fail(CCFV_EINVAL, "NULL context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drain_records(c);
+    memset(c->stats, 0, sizeof(c->stats));
+    for (int k = 0; k < CCFV_KSTAT_COUNT; k++) c->stats[k].kind = k;
+    c->prof = enable != 0;
+    return CCFV_OK;
+}
+
+int ccfv_profile_read(ccfv_ctx *c, ccfv_kernel_stat *out)
+{
+    if (!c || !out) return fail(CCFV_EINVAL, "NULL argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drain_records(c);
+    for (int k = 0; k < CCFV_KSTAT_COUNT; k++) {
+        out[k] = c->stats[k];
+        out[k].kind = k;
+    }
     return CCFV_OK;
 }
 

@@ -1,0 +1,90 @@
+"""C ABI checks that need no GPU: the library loads, exports every function
+declared in include/ccfv.h, and validates its arguments before touching a
+device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import ccfv
+from ccfv import types as T
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    src = open(os.path.join(ROOT, "include", "ccfv.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(ccfv_\w+)\s*\(", src, re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_the_boundary():
+    names = header_functions()
+    for n in ("ccfv_create", "ccfv_step", "ccfv_coarse", "ccfv_fine",
+              "ccfv_plain_solve", "ccfv_parareal_solve",
+              "ccfv_parareal_correction", "ccfv_last_error"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(ccfv.LIB_PATH)
+    for n in header_functions():
+        assert hasattr(lib, n), n
+    assert sorted(ccfv.exported_symbols()) == header_functions()
+
+
+def test_struct_layouts():
+    assert ctypes.sizeof(T.Problem) == 96
+    assert ctypes.sizeof(T.IC) == 8 + 16 * 8
+    assert ctypes.sizeof(T.ParaReport) == 16 + 64 * 8 + 24
+    assert ctypes.sizeof(T.KernelStat) == 24
+
+
+def test_reference_presets_match_the_c_side():
+    lib = ccfv.load()
+    p = T.Problem()
+    assert lib.ccfv_problem_reference(ctypes.byref(p), 128, 0.01) == 0
+    q = T.reference_problem(128, 0.01)
+    for f, _ in T.Problem._fields_:
+        a, b = getattr(p, f), getattr(q, f)
+        if f == "vel":
+            assert list(a) == list(b)
+        else:
+            assert a == b, f
+    ic = T.IC()
+    assert lib.ccfv_ic_reference(ctypes.byref(ic), 0.01) == 0
+    assert list(ic.p) == list(T.reference_ic().p)
+
+
+@pytest.mark.parametrize("kw,msg", [
+    ({"nx": 33}, "unsupported grid"),
+    ({"ny": 6}, "unsupported grid"),
+    ({"nu": -1.0}, "diffusion"),
+    ({"bc": 7}, "boundary"),
+    ({"vel_kind": 9}, "velocity"),
+    ({"x_hi": -2.0}, "box"),
+    ({"flags": T.FLAG_STEP3_POSTREMAP}, "reference"),
+])
+def test_create_rejects_bad_problems(kw, msg):
+    p = T.rotation_problem(64).copy(**kw)
+    with pytest.raises(ccfv.CcfvError) as ei:
+        ccfv.Context(p)
+    assert ei.value.code == T.CCFV_EINVAL
+    assert msg in str(ei.value)
+
+
+def test_create_without_device_fails_loudly():
+    if ccfv.device_count() > 0:
+        pytest.skip("a device is present")
+    with pytest.raises(ccfv.CcfvError) as ei:
+        ccfv.Context(T.rotation_problem(64))
+    assert ei.value.code == T.CCFV_EDEVICE
+
+
+def test_null_arguments_are_errors():
+    lib = ccfv.load()
+    assert lib.ccfv_step(None, 0.0, 0.1) == T.CCFV_EINVAL
+    assert lib.ccfv_plain_solve(None, 0.0, 1.0, 4) == T.CCFV_EINVAL
+    assert b"NULL" in lib.ccfv_last_error()

@@ -1,0 +1,339 @@
+"""Pins of the CPU oracle (oracle/ccfv_oracle.c) by mathematics.
+
+The reference ships no tests, fixtures or golden vectors (its repository is
+T2S4_Parareal.cu and a one-line README), so each oracle function is pinned by
+what the mathematics fixes: exactness on polynomials of the right degree (and
+failure one degree higher, so the check is not vacuous), closed-form Fourier
+symbols, dense linear algebra, invariants and the Parareal identities.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from ccfv import types as T
+
+
+# ---------------------------------------------------------------------------
+# helpers: exact cell averages of polynomials (unit cells, faces at integers)
+# ---------------------------------------------------------------------------
+def poly_avg(coef, a, b):
+    """Average of sum coef[k] x^k over [a, b]."""
+    P = np.polynomial.polynomial.Polynomial(coef).integ()
+    return (P(b) - P(a)) / (b - a)
+
+
+def poly_cells(coef, lo_index, n, d=1.0):
+    return np.array([poly_avg(coef, (lo_index + i) * d, (lo_index + i + 1) * d)
+                     for i in range(n)])
+
+
+# ---------------------------------------------------------------------------
+# reconstruction (T2S4_Parareal.cu:63-65, :260-304)
+# ---------------------------------------------------------------------------
+def test_face_values_exact_for_cubics(oracle):
+    rng = np.random.default_rng(1)
+    for _ in range(5):
+        c = rng.normal(size=4)
+        line = poly_cells(c, 0, 12)
+        for i in range(2, 10):
+            cl, cr, dl, c6 = oracle.ppm_coef(line, i)
+            P = np.polynomial.polynomial.Polynomial(c)
+            assert cl == pytest.approx(P(i), abs=1e-12)
+            assert cr == pytest.approx(P(i + 1), abs=1e-12)
+    # one degree higher is not reproduced
+    c = [0, 0, 0, 0, 1.0]
+    line = poly_cells(c, 0, 12)
+    cl = oracle.ppm_coef(line, 5)[0]
+    assert abs(cl - 5.0 ** 4) > 1e-3
+
+
+def test_antiderivative_integrates_the_parabola(oracle):
+    rng = np.random.default_rng(2)
+    line = rng.normal(size=10)
+    for i in range(2, 8):
+        cl, cr, dl, c6 = oracle.ppm_coef(line, i)
+        # the parabola a(x) = cl + x (dl + c6 (1 - x)) has a(0)=CL, a(1)=CR and
+        # mean C_i; its antiderivative is orc_intc
+        f = lambda x: cl + x * (dl + c6 * (1 - x))  # noqa: E731
+        for xi in (0.0, 0.3, 0.77, 1.0):
+            # Simpson's rule is exact for the quadratic integrand
+            ref = xi / 6 * (f(0.0) + 4 * f(xi / 2) + f(xi))
+            assert oracle.intc(c6, dl, cl, xi) == pytest.approx(ref, abs=1e-13)
+        assert f(0.0) == pytest.approx(cl)
+        assert f(1.0) == pytest.approx(cr)
+        assert oracle.intc(c6, dl, cl, 1.0) == pytest.approx(line[i], abs=1e-13)
+
+
+def test_ghost_closure_is_zero_slope_cubic(oracle):
+    """{Cm2, Cm1, CN, CNp1} (T2S4_Parareal.cu:275-286) are the averages of the
+    cubic with zero slope at the boundary face that matches three cells."""
+    rng = np.random.default_rng(3)
+    for _ in range(4):
+        a, c2, c3 = rng.normal(size=3)
+        coef = [a, 0.0, c2, c3]          # p'(0) = 0
+        line = poly_cells(coef, 0, 8)
+        g = oracle.ghosts(line)
+        assert g[1] == pytest.approx(poly_avg(coef, -1, 0), abs=1e-12)
+        assert g[0] == pytest.approx(poly_avg(coef, -2, -1), abs=1e-12)
+        # right end: mirror the cubic about the face x = 8
+        Pp = np.polynomial.polynomial.Polynomial
+        qcoef = Pp(coef)(Pp([8.0, -1.0])).coef   # q(x) = p(8 - x)
+        qline = poly_cells(qcoef, 0, 8)
+        gr = oracle.ghosts(qline)
+        assert gr[2] == pytest.approx(poly_avg(coef, -1, 0), abs=1e-11)
+        assert gr[3] == pytest.approx(poly_avg(coef, -2, -1), abs=1e-11)
+    # a linear profile (non-zero slope) is not reproduced
+    line = poly_cells([0.0, 1.0], 0, 8)
+    g = oracle.ghosts(line)
+    assert abs(g[1] - poly_avg([0.0, 1.0], -1, 0)) > 1e-3
+
+
+def test_foot_derivative_exact_for_quartics(oracle):
+    rng = np.random.default_rng(4)
+    d = 0.1
+    for _ in range(4):
+        c = rng.normal(size=5)
+        P = np.polynomial.polynomial.Polynomial(c)
+        line = poly_cells(c, 0, 12, d)
+        for iL in range(2, 10):
+            for xi in (0.0, 0.25, 0.5, 0.9):
+                x = (iL + xi) * d
+                D = oracle.deriv(line, iL, xi, d)
+                assert D == pytest.approx(P.deriv()(x), rel=1e-9, abs=1e-9)
+    c = [0, 0, 0, 0, 0, 1.0]
+    line = poly_cells(c, 0, 12, d)
+    P = np.polynomial.polynomial.Polynomial(c)
+    assert abs(oracle.deriv(line, 5, 0.3, d) - P.deriv()(5.3 * d)) > 1e-6
+
+
+def test_foot_derivative_continuous_across_faces(oracle):
+    """At xi=1 the weight of C_{iL-2} vanishes: the derivative from cell iL at
+    its right face equals the one from cell iL+1 at its left face."""
+    rng = np.random.default_rng(5)
+    line = rng.normal(size=16)
+    for iL in range(3, 12):
+        a = oracle.deriv(line, iL, 1.0, 0.5)
+        b = oracle.deriv(line, iL + 1, 0.0, 0.5)
+        assert a == pytest.approx(b, rel=1e-13, abs=1e-13)
+
+
+def test_derivative_quirk_only_changes_edge_cells(oracle):
+    rng = np.random.default_rng(6)
+    line = rng.normal(size=16)
+    n = len(line)
+    for iL in range(n):
+        a = oracle.deriv(line, iL, 0.4, 1.0, quirk=False)
+        b = oracle.deriv(line, iL, 0.4, 1.0, quirk=True)
+        if iL in (1, n - 2):
+            assert a != b
+        else:
+            assert a == b
+
+
+# ---------------------------------------------------------------------------
+# tridiagonal solves (T2S4_Parareal.cu:657-719)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [4, 9, 32])
+def test_thomas_matches_dense_solve(oracle, n):
+    rng = np.random.default_rng(n)
+    a, b = 1 / 12 - 0.3, 10 / 12 + 0.6
+    r = rng.normal(size=n)
+    A = np.diag(np.full(n, b)) + np.diag(np.full(n - 1, a), 1) + \
+        np.diag(np.full(n - 1, a), -1)
+    x = oracle.thomas(a, b, a, r)
+    np.testing.assert_allclose(x, np.linalg.solve(A, r), rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("n", [4, 9, 32, 64])
+def test_cyclic_matches_dense_solve(oracle, n):
+    rng = np.random.default_rng(100 + n)
+    for a, b in ((1 / 12 - 0.3, 10 / 12 + 0.6), (1 / 12, 10 / 12),
+                 (1 / 12 - 2.0, 10 / 12 + 4.0)):
+        r = rng.normal(size=n)
+        A = np.diag(np.full(n, b)) + np.diag(np.full(n - 1, a), 1) + \
+            np.diag(np.full(n - 1, a), -1)
+        A[0, -1] = A[-1, 0] = a
+        x = oracle.cyclic(a, b, r)
+        np.testing.assert_allclose(x, np.linalg.solve(A, r), rtol=1e-12,
+                                   atol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# whole sweeps and steps
+# ---------------------------------------------------------------------------
+def test_remap_exact_for_cubics_inside_the_box(oracle):
+    """nu = 0, closed box, constant displacement: interior outputs are the
+    exact averages of a cubic over the departure cells (the per-cell parabola
+    error is the same antisymmetric cubic in every cell, so the two partial
+    cells of a departure interval cancel it); quartics are not reproduced."""
+    n, lo, hi = 40, 0.0, 4.0
+    d = (hi - lo) / n
+    rng = np.random.default_rng(7)
+    coef = rng.normal(size=4)
+    line = poly_cells(coef, 0, n, d)
+    for delta in (0.37 * d, 1.6 * d, -2.3 * d):
+        faces = lo + d * np.arange(n + 1)
+        feet = faces - delta
+        out = oracle.line_sweep(line, feet, lo, hi, periodic=False, nu=0.0,
+                                h=0.1, theta=0.25)
+        for i in range(6, n - 6):
+            assert out[i] == pytest.approx(poly_avg(coef, feet[i], feet[i + 1]),
+                                           abs=1e-12)
+    # a quartic is not reproduced
+    coef3 = [0, 0, 0, 0, 1.0]
+    line3 = poly_cells(coef3, 0, n, d)
+    feet = lo + d * np.arange(n + 1) - 0.37 * d
+    out = oracle.line_sweep(line3, feet, lo, hi, periodic=False, nu=0.0, h=0.1,
+                            theta=0.25)
+    assert abs(out[20] - poly_avg(coef3, feet[20], feet[21])) > 1e-9
+
+
+def test_integer_cfl_is_a_cyclic_shift(oracle):
+    """nu = 0, constant velocity, every sub-step moving a whole number of cells:
+    the step is an exact cyclic shift of random data."""
+    n = 32
+    p = T.make_problem(n, n, bc=T.BC_PERIODIC, vel_kind=T.VEL_CONSTANT,
+                       vel=(1.0, -1.0), nu=0.0)
+    rng = np.random.default_rng(8)
+    u = rng.normal(size=n * n)
+    h = 4.0 / n   # x sub-steps move 2 cells each, y moves -4 cells
+    out = oracle.step(p, u, 0.0, h).reshape(n, n)
+    ref = np.roll(np.roll(u.reshape(n, n), 4, axis=0), -4, axis=1)
+    np.testing.assert_allclose(out, ref, rtol=0, atol=1e-12)
+
+
+def fourier_factor(theta, nu, h, d, kappa):
+    """Per-sweep amplification of a Fourier mode with V = 0: compact (1,10,1)/12
+    operator M, explicit half theta*nu*h/d^2 * (-1,16,-30,16,-1)/12 and implicit
+    half M - theta*nu*h/d^2 * (1,-2,1)."""
+    r2 = h / d ** 2
+    Mh = (10 + 2 * math.cos(kappa)) / 12
+    expl = 1 + theta * nu * r2 * (32 * math.cos(kappa) - 2 * math.cos(2 * kappa) - 30) / 12
+    impl = Mh - theta * nu * r2 * (2 * math.cos(kappa) - 2)
+    return Mh * expl / impl
+
+
+@pytest.mark.parametrize("k", [1, 3])
+def test_pure_diffusion_fourier_symbol(oracle, k):
+    n, nu, h = 32, 0.02, 1.0 / 64
+    p = T.make_problem(n, n, bc=T.BC_PERIODIC, vel_kind=T.VEL_CONSTANT,
+                       vel=(0.0, 0.0), nu=nu)
+    u = oracle.exact(p, T.make_ic(T.IC_SINSIN, (0.0, 0.0, 0.0)), 0.0)
+    if k != 1:
+        x = (np.arange(n) + 0.5) / n
+        u = np.outer(np.sin(2 * np.pi * k * x), np.sin(2 * np.pi * k * x)).ravel()
+    out = oracle.step(p, u, 0.0, h)
+    kap = 2 * np.pi * k / n
+    lam = fourier_factor(0.25, nu, h, 1 / n, kap) ** 2 * \
+        fourier_factor(0.5, nu, h, 1 / n, kap)
+    np.testing.assert_allclose(out, lam * u, rtol=0, atol=1e-13)
+
+
+def test_mass_conservation_and_constants_periodic(oracle):
+    p, ic = T.rotation_problem(64), T.rotation_ic()
+    u = oracle.exact(p, ic, 0.0)
+    m0 = u.sum()
+    v = oracle.plain(p, u, 0.0, 0.25, 64)
+    assert abs(v.sum() - m0) <= 1e-12 * abs(m0)
+    c = np.full(64 * 64, 0.7)
+    np.testing.assert_allclose(oracle.plain(p, c, 0.0, 0.1, 8), c, atol=1e-14)
+
+
+def test_rotating_gaussian_against_exact_solution(oracle):
+    """The reference problem (closed box, rotation, quirks on) against C_Exact
+    (T2S4_Parareal.cu:51-56) after a quarter turn."""
+    errs = []
+    for n in (64, 128):
+        p, ic = T.reference_problem(n, 0.001), T.reference_ic()
+        u = oracle.exact(p, ic, 0.0)
+        T_end = math.pi / 8  # quarter turn at omega = 4
+        nt = n // 2
+        v = oracle.plain(p, u, 0.0, T_end, nt)
+        ue = oracle.exact(p, ic, T_end)
+        errs.append(np.linalg.norm(v - ue) / np.linalg.norm(ue))
+    assert errs[1] < errs[0] / 4
+    assert errs[1] < 0.05
+
+
+def test_convergence_orders(oracle):
+    """Advection-diffusion of sin*sin with velocity (1, 0.5), nu = 0.01 against
+    the exact decaying translated solution.  At CFL 0.8 the corrected scheme is
+    third order (the fourth-order remap error accumulates over O(1/dx) steps);
+    the reference's third-sub-step derivative makes it first order; at a fixed
+    step count the spatial error is fourth order."""
+    def err(n, nt, flags):
+        p = T.advection_sinsin_problem(n, flags=flags)
+        ic = T.advection_sinsin_ic()
+        u = oracle.plain(p, oracle.exact(p, ic, 0.0), 0.0, 1.0, nt)
+        return np.sqrt(np.mean((u - oracle.exact(p, ic, 1.0)) ** 2))
+    e = [err(n, T.advection_sinsin_steps(n), 0) for n in (32, 64, 128)]
+    orders = [math.log2(e[i] / e[i + 1]) for i in range(2)]
+    assert all(2.8 < o < 3.4 for o in orders), orders
+    eq = [err(n, T.advection_sinsin_steps(n), T.FLAG_STEP3_POSTREMAP)
+          for n in (32, 64)]
+    assert 0.9 < math.log2(eq[0] / eq[1]) < 1.2
+    es = [err(n, 2560, 0) for n in (32, 64)]
+    assert 3.8 < math.log2(es[0] / es[1]) < 4.2
+
+
+# ---------------------------------------------------------------------------
+# initial data (T2S4_Parareal.cu:118-140)
+# ---------------------------------------------------------------------------
+def test_sinsin_cell_averages_closed_form(oracle):
+    from scipy import integrate
+    p = T.advection_sinsin_problem(16)
+    ic = T.advection_sinsin_ic()
+    t = 0.3
+    u = oracle.exact(p, ic, t).reshape(16, 16)
+    f = lambda y, x: (np.sin(2 * np.pi * (x - t)) * np.sin(2 * np.pi * (y - 0.5 * t))  # noqa: E731
+                      * np.exp(-8 * np.pi ** 2 * 0.01 * t))
+    for i, j in ((0, 0), (3, 7), (15, 9)):
+        v, _ = integrate.dblquad(f, i / 16, (i + 1) / 16, j / 16, (j + 1) / 16)
+        assert u[i, j] == pytest.approx(v * 256, abs=1e-12)
+
+
+def test_simpson_average_of_reference_gaussian(oracle):
+    from scipy import integrate
+    p, ic = T.reference_problem(32, 0.01), T.reference_ic()
+    u = oracle.exact(p, ic, 0.0).reshape(32, 32)
+    d = 2.0 / 32
+    f = lambda y, x: np.exp(-((x + 0.35) ** 2 + y ** 2) / 0.005)  # noqa: E731
+    for i, j in ((10, 16), (11, 15), (9, 17)):
+        x0, y0 = -1 + i * d, -1 + j * d
+        v, _ = integrate.dblquad(f, x0, x0 + d, y0, y0 + d)
+        # Simpson 3x3 is exact for bicubics; the Gaussian differs at O(d^4)
+        assert u[i, j] == pytest.approx(v / d ** 2, rel=2e-2)
+
+
+# ---------------------------------------------------------------------------
+# Parareal identities (T2S4_Parareal.cu:1250-1355)
+# ---------------------------------------------------------------------------
+def test_parareal_kmax_equal_slices_is_serial_fine(oracle):
+    p, ic = T.rotation_problem(32), T.rotation_ic()
+    u = oracle.exact(p, ic, 0.0)
+    r = oracle.parareal(p, u, 0.0, 0.5, 32, 4, 4)
+    s = oracle.plain(p, u, 0.0, 0.5, 32)
+    assert np.linalg.norm(r["u"] - s) <= 1e-13 * np.linalg.norm(s)
+
+
+def test_parareal_g_equal_f_converges_in_one_iteration(oracle):
+    p, ic = T.rotation_problem(32), T.rotation_ic()
+    u = oracle.exact(p, ic, 0.0)
+    r = oracle.parareal(p, u, 0.0, 0.5, 8, 8, 3)   # J = 1: G == F
+    assert r["increments"][0] <= 1e-15
+    s = oracle.plain(p, u, 0.0, 0.5, 8)
+    assert np.linalg.norm(r["u"] - s) <= 1e-14 * np.linalg.norm(s)
+
+
+def test_parareal_slices_up_to_k_are_exact(oracle):
+    p, ic = T.rotation_problem(32), T.rotation_ic()
+    u = oracle.exact(p, ic, 0.0)
+    ns, nt = 8, 32
+    r = oracle.parareal(p, u, 0.0, 1.0, nt, ns, 3, want_slices=True)
+    ser = [u]
+    for s in range(ns):
+        ser.append(oracle.plain(p, ser[-1], s / ns, (s + 1) / ns, nt // ns))
+    for s in range(4):
+        assert np.linalg.norm(r["slices"][s] - ser[s]) <= 1e-13 * np.linalg.norm(ser[s])
+    assert np.linalg.norm(r["slices"][ns] - ser[ns]) > 1e-10 * np.linalg.norm(ser[ns])
