@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="short run for counter collection: one 128 MiB "
+                         "calibration copy and one Parareal iteration")
     return ap.parse_args()
 
 
@@ -99,6 +102,21 @@ def main():
     from ccfv import types as T
 
     p, ic = T.rotation_problem(a.n), T.rotation_ic()
+    if a.probe:
+        # calibration: copy_kernel over 4096^2 doubles (128 MiB in, 128 MiB
+        # out) between two fields that are not cache resident
+        big = T.rotation_problem(4096)
+        c1, c2 = ccfv.Context(big), ccfv.Context(big)
+        c1.fill_exact(ic, 0.0)
+        junk = ccfv.Context(big)
+        junk.fill_exact(ic, 0.0)  # evict c1's field from the caches
+        c1.get_state_device(c2.state_ptr())
+        c1.sync()
+        ctx = ccfv.Context(p)
+        ctx.fill_exact(ic, 0.0)
+        r = ctx.parareal_solve(0.0, 1.0, a.nt, a.slices, 1, a.tol)
+        print(json.dumps({"probe": "ok", "iterations": r.iterations}))
+        return
     ctx = ccfv.Context(p)
     ctx.fill_exact(ic, 0.0)
     u0 = ctx.get_state()

@@ -159,6 +159,10 @@ int ccfv_get_state(ccfv_ctx *ctx, double *host);
  * nx*ny doubles on the context's device). */
 int ccfv_set_state_device(ccfv_ctx *ctx, const double *dev);
 int ccfv_get_state_device(ccfv_ctx *ctx, double *dev);
+/* Device address of the context's state (nx*ny doubles, reference layout),
+ * for callers that exchange fields in HBM (e.g. wrap it as a tensor).  Valid
+ * until ccfv_destroy. */
+int ccfv_state_device_ptr(ccfv_ctx *ctx, void **dev);
 /* Cell averages of an initial/exact solution at time t into the state
  * (replaces get_exact_solution_on_gpu, :118-140).  Computed on the device. */
 int ccfv_fill_exact(ccfv_ctx *ctx, const ccfv_ic *ic, double t);
@@ -196,6 +200,21 @@ int ccfv_parareal_correction(ccfv_ctx *ctx, const double *g_new,
 
 /* Block until all work queued on the context's stream is done. */
 int ccfv_sync(ccfv_ctx *ctx);
+
+/* ---- launch shapes ------------------------------------------------------
+ * Each sweep launch stages W lines per workgroup and sweeps them with NW
+ * wavefronts.  Shapes are kept per sweep direction for one field (plain
+ * sweep, Parareal coarse chain) and for batches (fine propagators).
+ * tiles[8] = {x1.W, x1.NW, xb.W, xb.NW, y1.W, y1.NW, yb.W, yb.NW}.
+ * ccfv_autotune times every valid shape with hipEvents on the context's
+ * stream (one field and batch_fields fields, step size h; the state is
+ * copied into scratch fields, the context state is untouched) and keeps the
+ * fastest.  It has no counterpart in the reference, whose launch shape is a
+ * fixed 128-thread 1-D grid (T2S4_Parareal.cu:1073-1080). */
+int ccfv_get_tiles(ccfv_ctx *ctx, int32_t tiles[8]);
+int ccfv_set_tiles(ccfv_ctx *ctx, const int32_t tiles[8]);
+int ccfv_autotune(ccfv_ctx *ctx, int32_t batch_fields, double h,
+                  int32_t tiles_out[8]);
 
 /* ---- kernel timing (instrumentation; no counterpart in the reference,
  * which only reads clock() around the whole program, :1137, :1357) -------

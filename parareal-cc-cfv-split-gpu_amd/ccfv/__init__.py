@@ -44,6 +44,8 @@ _SIGS = {
     "ccfv_get_state": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "ccfv_set_state_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "ccfv_get_state_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "ccfv_state_device_ptr": (ctypes.c_int, [ctypes.c_void_p,
+                                             ctypes.POINTER(ctypes.c_void_p)]),
     "ccfv_fill_exact": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(IC),
                                        ctypes.c_double]),
     "ccfv_exact_to_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(IC),
@@ -64,6 +66,10 @@ _SIGS = {
     "ccfv_parareal_correction": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4
                                  + [ctypes.c_size_t]),
     "ccfv_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "ccfv_get_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    "ccfv_set_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    "ccfv_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double,
+                                     ctypes.POINTER(ctypes.c_int32)]),
     "ccfv_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "ccfv_profile_read": (ctypes.c_int, [ctypes.c_void_p,
                                          ctypes.POINTER(types.KernelStat)]),
@@ -164,6 +170,21 @@ class Context:
         _check(load().ccfv_get_state(self._h, _ptr(out)), "ccfv_get_state")
         return out
 
+    def state_ptr(self) -> int:
+        """Device address of the state (nx*ny doubles)."""
+        v = ctypes.c_void_p()
+        _check(load().ccfv_state_device_ptr(self._h, ctypes.byref(v)),
+               "ccfv_state_device_ptr")
+        return int(v.value)
+
+    def set_state_device(self, ptr: int):
+        _check(load().ccfv_set_state_device(self._h, ctypes.c_void_p(ptr)),
+               "ccfv_set_state_device")
+
+    def get_state_device(self, ptr: int):
+        _check(load().ccfv_get_state_device(self._h, ctypes.c_void_p(ptr)),
+               "ccfv_get_state_device")
+
     def fill_exact(self, ic: IC, t: float = 0.0):
         _check(load().ccfv_fill_exact(self._h, ctypes.byref(ic), t),
                "ccfv_fill_exact")
@@ -198,6 +219,21 @@ class Context:
 
     def sync(self):
         _check(load().ccfv_sync(self._h), "ccfv_sync")
+
+    def get_tiles(self) -> list:
+        t = (ctypes.c_int32 * 8)()
+        _check(load().ccfv_get_tiles(self._h, t), "ccfv_get_tiles")
+        return list(t)
+
+    def set_tiles(self, tiles):
+        t = (ctypes.c_int32 * 8)(*tiles)
+        _check(load().ccfv_set_tiles(self._h, t), "ccfv_set_tiles")
+
+    def autotune(self, batch_fields: int, h: float) -> list:
+        """Time every valid launch shape and keep the fastest (see ccfv.h)."""
+        t = (ctypes.c_int32 * 8)()
+        _check(load().ccfv_autotune(self._h, batch_fields, h, t), "ccfv_autotune")
+        return list(t)
 
     def profile_enable(self, on: bool = True):
         _check(load().ccfv_profile_enable(self._h, int(on)), "ccfv_profile_enable")

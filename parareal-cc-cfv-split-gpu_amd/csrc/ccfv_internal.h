@@ -22,7 +22,13 @@ enum Epi { EPI_NONE = 0, EPI_INIT = 1, EPI_CORRECT = 2 };
 // coefficients and the Thomas factors of the constant matrix (identical for
 // every line of a sweep, so they are computed once on the host).
 struct SolveTable {
-    double *dev = nullptr;  // [mneg(n) | den(n) | pneg(n) | z(n)]
+    // [mneg(n) | den(n) | pneg(n) | z(n) | mf(64) | mb(64)]: Thomas factors
+    // of the constant matrix and the per-lane chunk products of mneg / pneg
+    double *dev = nullptr;
+    // uniform-displacement path: per line the cell shift S (as double) and the
+    // five filter weights; fast_ok when every line passed the edge guard
+    double *line_dev = nullptr;
+    bool fast_ok = false;
     double ca = 0, cb = 0;  // off-diagonal, diagonal
     double diag0 = 0;       // first diagonal entry of the factored matrix
     double coef = 0;        // explicit diffusion factor (theta*nu)*(h/d)
@@ -39,14 +45,13 @@ struct SweepArgs {
     int nl;             // lines (cells in the other direction)
     int ld;             // row length of the field (ny)
     int quirk;          // reference ghost selection in the derivative stencil
-    int uniform;        // along-line velocity independent of the along-line
-                        // coordinate (uniform foot offset on a line)
     int sf, sb;         // lane-scan doubling steps, forward / backward
     double lo, hi, d;   // along-line box and cell width
     double olo, od;     // other-direction box start and cell width
     double eps;         // remap clip epsilon (closed box)
     double coef, ca, diag0, vl, smden;
     const double *tab;
+    const double *lineS, *lineK;  // uniform path: shift and weights per line
     int vel_kind;
     double v0, v1, v2;  // velocity parameters
     // time: field f of the launch sits at tn = t0 + (field0 + f)*tDT, the
@@ -72,10 +77,16 @@ struct Tile {
 int chunk_for(int n);
 bool supported_n(int n);
 
-// kernels (ccfv_kernels.hip)
-hipError_t launch_sweep(Dir dir, int bc, bool post, Epi epi, const SweepArgs &a,
-                        int nfields, Tile tile, hipStream_t s);
-size_t sweep_lds_bytes(int n, Tile tile, bool post);
+// general sweeps (ccfv_sweep_generic.hip)
+hipError_t launch_sweep_generic(Dir dir, int bc, bool post, Epi epi,
+                                const SweepArgs &a, int nfields, Tile tile,
+                                hipStream_t s);
+size_t sweep_lds_bytes_generic(int n, Tile tile);
+// uniform-displacement sweeps (ccfv_sweep_fast.hip); periodic only
+hipError_t launch_sweep_fast(Dir dir, Epi epi, const SweepArgs &a, int nfields,
+                             Tile tile, hipStream_t s);
+size_t sweep_lds_bytes_fast(int n, Tile tile);
+// element-wise kernels (ccfv_misc.hip)
 hipError_t launch_correction(const double *g_new, const double *g_old,
                              const double *f, double *u, size_t n,
                              double *partial, int nblocks, hipStream_t s);

@@ -1,0 +1,547 @@
+// ccfv_line.h — device building blocks of one line sweep, shared by the
+// general and the uniform-displacement sweep kernels.
+//
+// A line is one x- or y-line of the field staged in LDS; one wavefront sweeps
+// it with `CH` contiguous cells per lane.  The stages restate
+// T2S4_Parareal.cu: feet (:203-258), PPM coefficients (:260-304), remap
+// (:306-438), foot derivative (:440-547), RHS (:564-613), compact operator
+// (:616-654) and the tridiagonal implicit half (:657-719).
+#pragma once
+
+#include "ccfv_feet.h"
+#include "ccfv_internal.h"
+
+namespace ccfv {
+
+constexpr int BC_REF = CCFV_BC_REFERENCE;
+constexpr int BC_PER = CCFV_BC_PERIODIC;
+
+// LDS position of element e of a line: one pad slot every 8 elements keeps
+// both the chunked (lane*CH + k) and the strided (lane + 64k) access
+// patterns free of bank conflicts.
+__device__ __forceinline__ int pidx(int e) { return e + (e >> 3); }
+__host__ __device__ inline int line_pitch(int n) { return n + (n >> 3) + 1; }
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi)
+{
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+struct Times {
+    double hi, lo;
+};
+
+// sampling times of the velocity for the sub-step of field blockIdx.y:
+// x sub-step 1 over [tn, tn+h/2], y over [tn, tn+h], x sub-step 3 over
+// [tn+h/2, tn+h]; tn = t0 + n*DT (+ j*dt inside the fine propagator, :981)
+__device__ __forceinline__ Times sub_times(const SweepArgs &a)
+{
+#pragma clang fp contract(off)
+    double tn = a.t0 + (double)(a.field0 + (int)blockIdx.y) * a.tDT;
+    double tl = tn + a.jdt;
+    Times t;
+    if (a.substep == 1) {
+        t.lo = tl;
+        t.hi = tl + 0.5 * a.h;
+    } else if (a.substep == 3) {
+        t.lo = tl + 0.5 * a.h;
+        t.hi = tl + a.h;
+    } else {
+        t.lo = tl;
+        t.hi = tl + a.h;
+    }
+    return t;
+}
+
+// foot of face q of the line with corner coordinates o0, o1
+template <int DIR>
+__device__ __forceinline__ double face_foot(const SweepArgs &a, int q,
+                                            double o0, double o1)
+{
+    Times t = sub_times(a);
+    VelParams v{a.vel_kind, a.v0, a.v1, a.v2};
+    return face_foot<DIR>(v, a.lo, a.d, a.tau, t.hi, t.lo, q, o0, o1);
+}
+
+__device__ __forceinline__ int cell_of(const SweepArgs &a, double x, double &xi)
+{
+    return cell_of(a.lo, a.d, a.n, x, xi);
+}
+
+// ---------------------------------------------------------------------------
+// reconstruction
+// ---------------------------------------------------------------------------
+// zero-slope cubic ghosts {Cm2, Cm1, CN, CNp1} (T2S4_Parareal.cu:275-286)
+__device__ __forceinline__ void ghosts(const double *cs, int n, double g[4])
+{
+    const double e = 1.0 / 11.0;
+    double c0 = cs[pidx(0)], c1 = cs[pidx(1)], c2 = cs[pidx(2)];
+    double a0 = cs[pidx(n - 3)], a1 = cs[pidx(n - 2)], a2 = cs[pidx(n - 1)];
+    g[2] = e * (-a0 + 3 * a1 + 9 * a2);
+    g[3] = e * (-15 * a0 + 56 * a1 - 30 * a2);
+    g[1] = e * (-c2 + 3 * c1 + 9 * c0);
+    g[0] = e * (-15 * c2 + 56 * c1 - 30 * c0);
+}
+
+// five cell averages around cell m: E[m-2..m+2]; m in [0,n) (periodic: m is
+// already wrapped; the closed box substitutes the ghosts)
+template <int BC>
+__device__ __forceinline__ void window(const double *cs, int n,
+                                       const double g[4], int m, double w[5])
+{
+#pragma unroll
+    for (int s = 0; s < 5; s++) {
+        int k = m - 2 + s;
+        if (BC == BC_PER) {
+            k = k < 0 ? k + n : (k >= n ? k - n : k);
+            w[s] = cs[pidx(k)];
+        } else {
+            double v = cs[pidx(clampi(k, 0, n - 1))];
+            w[s] = k < 0 ? (k == -1 ? g[1] : g[0])
+                         : (k >= n ? (k == n ? g[2] : g[3]) : v);
+        }
+    }
+}
+
+struct Ppm {
+    double cl, dl, c6;
+};
+
+// PPM parabola of the centre cell of w (T2S4_Parareal.cu:296-300)
+__device__ __forceinline__ Ppm ppm(const double w[5])
+{
+    const double s12 = 7.0 / 12.0, o12 = 1.0 / 12.0;
+    double cl = s12 * (w[2] + w[1]) - o12 * (w[3] + w[0]);
+    double cr = s12 * (w[3] + w[2]) - o12 * (w[4] + w[1]);
+    Ppm p;
+    p.cl = cl;
+    p.dl = cr - cl;
+    p.c6 = 6.0 * (w[2] - 0.5 * (cl + cr));
+    return p;
+}
+
+// antiderivative of the parabola (T2S4_Parareal.cu:63-65)
+__device__ __forceinline__ double intc(const Ppm &p, double xi)
+{
+    return -1.0 / 3 * p.c6 * (xi * xi * xi) + 0.5 * (p.dl + p.c6) * (xi * xi) +
+           p.cl * xi;
+}
+
+// derivative of the quartic through the five averages (T2S4_Parareal.cu:481-486)
+__device__ __forceinline__ double deriv(const double w[5], double xi,
+                                        double inv12d)
+{
+    double x2 = xi * xi, x3 = x2 * xi;
+    return inv12d * ((2.0 * x3 - 6.0 * x2 + 3.0 * xi + 1.0) * w[0] +
+                     (-8.0 * x3 + 18.0 * x2 + 6.0 * xi - 15.0) * w[1] +
+                     (12.0 * x3 - 18.0 * x2 - 24.0 * xi + 15.0) * w[2] +
+                     (-8.0 * x3 + 6.0 * x2 + 18.0 * xi - 1.0) * w[3] +
+                     (2.0 * x3 - 3.0 * xi) * w[4]);
+}
+
+__device__ __forceinline__ int wrapn(int k, int n)
+{
+    k %= n;
+    return k < 0 ? k + n : k;
+}
+
+struct Face {
+    double xb;        // raw foot
+    double xiL;       // offset of the (left-clipped) foot in its cell
+    int mL, mR;       // cells of the foot as left / right end of a cell
+    double ivL, icL;  // I(xi), I(1)-I(xi) as left end
+    double ivR;       // I(xi) as right end
+    double D;         // derivative at the foot
+};
+
+// Closed box foot derivative: zero outside the box, else clamp to
+// [lo+1e-15, hi-1e-15] (T2S4_Parareal.cu:457-490, :514-543).
+template <int BC>
+__device__ __forceinline__ double deriv_ref(const SweepArgs &a,
+                                            const double *src,
+                                            const double g[4], double xb,
+                                            double inv12d)
+{
+#pragma clang fp contract(off)
+    if (!(xb >= a.lo && xb <= a.hi)) return 0.0;
+    double xd = fmin(fmax(xb, a.lo + 1e-15), a.hi - 1e-15);
+    double xi;
+    int m = cell_of(a, xd, xi);
+    int md = clampi(m, 0, a.n - 1);
+    if (md != m) xi = (xd - (a.lo + a.d * md)) / a.d;
+    double w[5];
+    window<BC_REF>(src, a.n, g, md, w);
+    if (a.quirk) {  // :476-479 — iL==Nx-2 takes CNp1, iL==1 takes Cm2
+        if (md > a.n - 3) w[4] = g[3];
+        if (md < 2) w[0] = g[0];
+    }
+    return deriv(w, xi, inv12d);
+}
+
+template <int DIR, int BC, bool POST>
+__device__ __forceinline__ Face eval_face(const SweepArgs &a, const double *cs,
+                                          const double g[4], int q, double o0,
+                                          double o1, double inv12d)
+{
+    Face f;
+    const int n = a.n;
+    if (BC == BC_PER) {
+        // face n is face 0 one period on (exact periodicity of the fluxes)
+        int qq = q == n ? 0 : q;
+        f.xb = face_foot<DIR>(a, qq, o0, o1);
+        double xi;
+        int m = cell_of(a, f.xb, xi);
+        int mw = wrapn(m, n);
+        double w[5];
+        window<BC_PER>(cs, n, g, mw, w);
+        Ppm p = ppm(w);
+        double iv = intc(p, xi);
+        f.ivL = f.ivR = iv;
+        f.icL = intc(p, 1.0) - iv;
+        f.xiL = xi;
+        f.mL = f.mR = q == n ? m + n : m;
+        f.D = POST ? 0.0 : deriv(w, xi, inv12d);
+    } else {
+        f.xb = face_foot<DIR>(a, q, o0, o1);
+        double xl, xr;
+        {
+#pragma clang fp contract(off)
+            xl = fmax(f.xb, a.lo + a.eps);
+            xr = fmin(f.xb, a.hi - a.eps);
+        }
+        double xi;
+        int m = clampi(cell_of(a, xl, xi), 0, n - 1);
+        {
+#pragma clang fp contract(off)
+            xi = (xl - (a.lo + a.d * m)) / a.d;
+        }
+        double w[5];
+        window<BC_REF>(cs, n, g, m, w);
+        Ppm p = ppm(w);
+        f.mL = m;
+        f.xiL = xi;
+        f.ivL = intc(p, xi);
+        f.icL = intc(p, 1.0) - f.ivL;
+        if (xr == xl) {
+            f.mR = m;
+            f.ivR = f.ivL;
+        } else {
+            double xi2;
+            int m2 = clampi(cell_of(a, xr, xi2), 0, n - 1);
+            {
+#pragma clang fp contract(off)
+                xi2 = (xr - (a.lo + a.d * m2)) / a.d;
+            }
+            double w2[5];
+            window<BC_REF>(cs, n, g, m2, w2);
+            Ppm p2 = ppm(w2);
+            f.mR = m2;
+            f.ivR = intc(p2, xi2);
+        }
+        f.D = POST ? 0.0 : deriv_ref<BC>(a, cs, g, f.xb, inv12d);
+    }
+    return f;
+}
+
+// conservative remap of one cell between its feet (T2S4_Parareal.cu:319-344)
+template <int BC>
+__device__ __forceinline__ double remap_cell(const SweepArgs &a,
+                                             const double *cs, const Face &L,
+                                             const Face &R, bool &bad)
+{
+    const int n = a.n;
+    if (BC == BC_REF) {
+        if (R.xb < L.xb) bad = true;
+        if (L.xb > a.hi || R.xb < a.lo || L.xb == R.xb) return 0.0;
+        if (R.mR == L.mL) return R.ivR - L.ivL;
+        double s = 0.0;
+        for (int m = L.mL + 1; m <= R.mR - 1; m++) s += cs[pidx(m)];
+        return L.icL + s + R.ivR;
+    } else {
+        int mL = L.mL, mR = R.mR;
+        if (mR < mL || (mR == mL && R.xiL < L.xiL) || mR - mL > n) {
+            bad = true;
+            return 0.0;
+        }
+        if (mR == mL) return R.ivR - L.ivL;
+        int w = (mL + 1) % n;
+        w = w < 0 ? w + n : w;
+        double s = 0.0;
+        for (int m = mL + 1; m <= mR - 1; m++) {
+            s += cs[pidx(w)];
+            w = w + 1 == n ? 0 : w + 1;
+        }
+        return L.icL + s + R.ivR;
+    }
+}
+
+
+// RHS b of the cells of this lane by the general per-face path; b goes to the
+// lane's own slots of `aux` (an LDS line used as an indexable register file).
+template <int DIR, int BC>
+__device__ void rhs_general(const SweepArgs &a, const double *cs,
+                            const double g[4], double *aux, int base, int ch,
+                            double o0, double o1, double inv12d, bool post)
+{
+    bool bad = false;
+    Face fl = post ? eval_face<DIR, BC, true>(a, cs, g, base, o0, o1, inv12d)
+                   : eval_face<DIR, BC, false>(a, cs, g, base, o0, o1, inv12d);
+#pragma unroll 1
+    for (int k = 0; k < ch; k++) {
+        Face fr = post ? eval_face<DIR, BC, true>(a, cs, g, base + k + 1, o0, o1,
+                                                  inv12d)
+                       : eval_face<DIR, BC, false>(a, cs, g, base + k + 1, o0, o1,
+                                                   inv12d);
+        double cbar = remap_cell<BC>(a, cs, fl, fr, bad);
+        aux[pidx(base + k)] = post ? cbar : cbar + a.coef * (fr.D - fl.D);
+        fl = fr;
+    }
+    if (bad) *a.flag = 1;
+}
+
+// ---------------------------------------------------------------------------
+// block reduction of two doubles into partial[2*blockIdx.x ...]
+// ---------------------------------------------------------------------------
+__device__ inline void block_reduce2(double s0, double s1, double *partial, int slot)
+{
+    __shared__ double red[2][16];
+    for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_down(s0, off);
+        s1 += __shfl_down(s1, off);
+    }
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        red[0][wave] = s0;
+        red[1][wave] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t0 = 0.0, t1 = 0.0;
+        int nw = blockDim.x >> 6;
+        for (int w = 0; w < nw; w++) {
+            t0 += red[0][w];
+            t1 += red[1][w];
+        }
+        partial[2 * slot] = t0;
+        partial[2 * slot + 1] = t1;
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// compact operator and implicit half, shared by both paths
+// ---------------------------------------------------------------------------
+// b: RHS of this lane's CH cells.  Tables: mneg = -a*den, den = Thomas
+// pivots' inverses, pneg = -p, z = T^{-1} u (Sherman-Morrison), mf/mb =
+// per-lane chunk products of mneg/pneg (identity 1 on idle lanes).  LDSTAB:
+// mneg..z are LDS lines (pidx layout) rather than global arrays.  The result
+// overwrites the lane's cells of cs.
+template <int BC, int CH, bool LDSTAB>
+__device__ __forceinline__ void solve_tail(const SweepArgs &a, double (&b)[CH],
+                                           int lane, bool active,
+                                           const double *mneg, const double *den,
+                                           const double *pneg, const double *zt,
+                                           const double *mf, const double *mb,
+                                           double *cs)
+{
+    const int n = a.n, NL = n / CH, base = lane * CH;
+#define CCFV_TAB(t, e) (LDSTAB ? (t)[pidx(e)] : (t)[e])
+    // (b_{i-1} + 10 b_i + b_{i+1}) / 12, neighbours across lanes by shuffle;
+    // zero ghosts in the closed box (:624-631), wrap-around when periodic
+    double bprev = __shfl_up(b[CH - 1], 1);
+    double bnext = __shfl_down(b[0], 1);
+    double wrap_last = __shfl(b[CH - 1], NL - 1);
+    double wrap_first = __shfl(b[0], 0);
+    if (lane == 0) bprev = BC == BC_PER ? wrap_last : 0.0;
+    if (lane == NL - 1) bnext = BC == BC_PER ? wrap_first : 0.0;
+    double r[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        double bn = k + 1 < CH ? b[k + 1] : bnext;
+        double bp = k > 0 ? b[k - 1] : bprev;
+        r[k] = (bn + 10.0 * b[k] + bp) * (1.0 / 12.0);
+    }
+
+    // forward pass q_e = r_e den_e + mneg_e q_{e-1}: chunk pass, lane scan
+    // (a.sf doubling steps reach every factor product above 1e-18), replay
+    double M = mf[lane], Q = 0.0;
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            int e = base + k;
+            double c = r[k] * CCFV_TAB(den, e);
+            r[k] = c;
+            Q = c + CCFV_TAB(mneg, e) * Q;
+        }
+    }
+    for (int st = 0, off = 1; st < a.sf; st++, off <<= 1) {
+        double Mp = __shfl_up(M, off), Qp = __shfl_up(Q, off);
+        if (lane >= off) {
+            Q = M * Qp + Q;
+            M = M * Mp;
+        }
+    }
+    double carry = __shfl_up(Q, 1);
+    if (lane == 0) carry = 0.0;
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            carry = r[k] + CCFV_TAB(mneg, base + k) * carry;
+            r[k] = carry;
+        }
+    }
+
+    // back substitution x_e = q_e + pneg_e x_{e+1}
+    M = mb[lane];
+    Q = 0.0;
+    if (active) {
+#pragma unroll
+        for (int k = CH - 1; k >= 0; k--) Q = r[k] + CCFV_TAB(pneg, base + k) * Q;
+    }
+    for (int st = 0, off = 1; st < a.sb; st++, off <<= 1) {
+        double Mn = __shfl_down(M, off), Qn = __shfl_down(Q, off);
+        if (lane + off < 64) {
+            Q = M * Qn + Q;
+            M = M * Mn;
+        }
+    }
+    carry = __shfl_down(Q, 1);
+    if (lane == 63) carry = 0.0;
+    if (active) {
+#pragma unroll
+        for (int k = CH - 1; k >= 0; k--) {
+            carry = r[k] + CCFV_TAB(pneg, base + k) * carry;
+            r[k] = carry;
+        }
+    }
+
+    if (BC == BC_PER) {
+        // Sherman-Morrison: x = y - ((y_0 + vl y_{n-1}) / (1 + v.z)) z
+        double y0 = __shfl(r[0], 0);
+        double yn = __shfl(r[CH - 1], NL - 1);
+        double fac = (y0 + a.vl * yn) / a.smden;
+        if (active) {
+#pragma unroll
+            for (int k = 0; k < CH; k++) r[k] = r[k] - fac * CCFV_TAB(zt, base + k);
+        }
+    }
+#undef CCFV_TAB
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) cs[pidx(base + k)] = r[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// tiles: global <-> LDS
+// ---------------------------------------------------------------------------
+// x lines are columns j0..j0+W-1 of the row-major field: rows of W doubles
+// are read and written coalesced (double2) and transposed through LDS.
+// W is a power of two (enforced by the host), so idx splits by shifts.
+__device__ __forceinline__ int log2i(int v) { return 31 - __builtin_clz(v); }
+
+__device__ __forceinline__ void load_tile_x(const double *in, int ld, int n,
+                                            int j0, int W, double *lds, int LP)
+{
+    const int hw = W >> 1, sh = log2i(hw);
+    for (int idx = threadIdx.x; idx < hw * n; idx += blockDim.x) {
+        int c2 = idx & (hw - 1), i = idx >> sh;
+        double2 v = *reinterpret_cast<const double2 *>(in + (size_t)i * ld + j0 +
+                                                       2 * c2);
+        lds[(2 * c2) * LP + pidx(i)] = v.x;
+        lds[(2 * c2 + 1) * LP + pidx(i)] = v.y;
+    }
+}
+
+__device__ __forceinline__ void store_tile_x(double *out, int ld, int n, int j0,
+                                             int W, const double *lds, int LP)
+{
+    const int hw = W >> 1, sh = log2i(hw);
+    for (int idx = threadIdx.x; idx < hw * n; idx += blockDim.x) {
+        int c2 = idx & (hw - 1), i = idx >> sh;
+        double2 v;
+        v.x = lds[(2 * c2) * LP + pidx(i)];
+        v.y = lds[(2 * c2 + 1) * LP + pidx(i)];
+        *reinterpret_cast<double2 *>(out + (size_t)i * ld + j0 + 2 * c2) = v;
+    }
+}
+
+// Parareal epilogue of the last x sweep of a coarse step: g is G(U_n^{k+1}).
+//   EPI_INIT   : G cache <- g, U[n+1] <- g
+//   EPI_CORRECT: U[n+1] <- g + (F - G_old) (T2S4_Parareal.cu:779),
+//                G cache <- g, block partials of |dU|^2 and |U|^2
+template <int EPI>
+__device__ __forceinline__ void epilogue_x(const SweepArgs &a, int n, int j0,
+                                           int W, const double *lds, int LP)
+{
+    const int hw = W >> 1, sh = log2i(hw);
+    double s0 = 0.0, s1 = 0.0;
+    for (int idx = threadIdx.x; idx < hw * n; idx += blockDim.x) {
+        int c2 = idx & (hw - 1), i = idx >> sh;
+        size_t gi = (size_t)i * a.ld + j0 + 2 * c2;
+        double2 g;
+        g.x = lds[(2 * c2) * LP + pidx(i)];
+        g.y = lds[(2 * c2 + 1) * LP + pidx(i)];
+        if (EPI == EPI_INIT) {
+            *reinterpret_cast<double2 *>(a.g_cache + gi) = g;
+            *reinterpret_cast<double2 *>(a.u_next + gi) = g;
+        } else {
+            double2 go = *reinterpret_cast<const double2 *>(a.g_cache + gi);
+            double2 fv = *reinterpret_cast<const double2 *>(a.f_in + gi);
+            double2 uo = *reinterpret_cast<const double2 *>(a.u_next + gi);
+            double2 un;
+            un.x = g.x + (fv.x - go.x);
+            un.y = g.y + (fv.y - go.y);
+            *reinterpret_cast<double2 *>(a.g_cache + gi) = g;
+            *reinterpret_cast<double2 *>(a.u_next + gi) = un;
+            double d0 = un.x - uo.x, d1 = un.y - uo.y;
+            s0 += d0 * d0 + d1 * d1;
+            s1 += un.x * un.x + un.y * un.y;
+        }
+    }
+    if (EPI == EPI_CORRECT) block_reduce2(s0, s1, a.partial, blockIdx.x);
+}
+
+// y lines are rows i0..i0+W-1 (contiguous)
+// row index and offset of element idx of a W x hn tile (shifts when hn is a
+// power of two)
+__device__ __forceinline__ void split_row(int idx, int hn, int &r, int &j2)
+{
+    if ((hn & (hn - 1)) == 0) {
+        int sh = log2i(hn);
+        r = idx >> sh;
+        j2 = idx & (hn - 1);
+    } else {
+        r = idx / hn;
+        j2 = idx - r * hn;
+    }
+}
+
+__device__ __forceinline__ void load_tile_y(const double *in, int ld, int n,
+                                            int W, double *lds, int LP)
+{
+    const int hn = n >> 1;
+    for (int idx = threadIdx.x; idx < W * hn; idx += blockDim.x) {
+        int r, j2;
+        split_row(idx, hn, r, j2);
+        double2 v = *reinterpret_cast<const double2 *>(in + (size_t)r * ld + 2 * j2);
+        lds[r * LP + pidx(2 * j2)] = v.x;
+        lds[r * LP + pidx(2 * j2 + 1)] = v.y;
+    }
+}
+
+__device__ __forceinline__ void store_tile_y(double *out, int ld, int n, int W,
+                                             const double *lds, int LP)
+{
+    const int hn = n >> 1;
+    for (int idx = threadIdx.x; idx < W * hn; idx += blockDim.x) {
+        int r, j2;
+        split_row(idx, hn, r, j2);
+        double2 v;
+        v.x = lds[r * LP + pidx(2 * j2)];
+        v.y = lds[r * LP + pidx(2 * j2 + 1)];
+        *reinterpret_cast<double2 *>(out + (size_t)r * ld + 2 * j2) = v;
+    }
+}
+
+}  // namespace ccfv

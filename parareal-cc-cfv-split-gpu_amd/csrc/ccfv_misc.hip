@@ -1,0 +1,255 @@
+// ccfv_misc.hip — element-wise kernels of the Parareal driver, reductions and
+// the initial / exact cell averages.
+#include "ccfv_line.h"
+
+namespace ccfv {
+
+int chunk_for(int n)
+{
+    int ch = 1;
+    while (ch * 64 < n) ch *= 2;
+    return ch;
+}
+
+bool supported_n(int n)
+{
+    if (n < 8 || n > 2048 || (n & 1)) return false;
+    return n % chunk_for(n) == 0;
+}
+
+// ---------------------------------------------------------------------------
+// element-wise kernels
+// ---------------------------------------------------------------------------
+// U = G_new + (F - G_old), with increment partials (T2S4_Parareal.cu:771-782)
+__global__ __launch_bounds__(256) void correction_kernel(
+    const double *g_new, const double *g_old, const double *f, double *u,
+    size_t n, double *partial)
+{
+    double s0 = 0.0, s1 = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        double un = g_new[i] + (f[i] - g_old[i]);
+        if (partial) {
+            double d = un - u[i];
+            s0 += d * d;
+            s1 += un * un;
+        }
+        u[i] = un;
+    }
+    if (partial) block_reduce2(s0, s1, partial, blockIdx.x);
+}
+
+hipError_t launch_correction(const double *g_new, const double *g_old,
+                             const double *f, double *u, size_t n,
+                             double *partial, int nblocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(correction_kernel, dim3(nblocks), dim3(256), 0, s, g_new,
+                       g_old, f, u, n, partial);
+    return hipGetLastError();
+}
+
+// fixed-order sum of count (num, den) pairs
+__global__ __launch_bounds__(256) void reduce_partials_kernel(
+    const double *partial, int count, double *out2)
+{
+    __shared__ double r0[256], r1[256];
+    double s0 = 0.0, s1 = 0.0;
+    for (int i = threadIdx.x; i < count; i += 256) {
+        s0 += partial[2 * i];
+        s1 += partial[2 * i + 1];
+    }
+    r0[threadIdx.x] = s0;
+    r1[threadIdx.x] = s1;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            r0[threadIdx.x] += r0[threadIdx.x + w];
+            r1[threadIdx.x] += r1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out2[0] = r0[0];
+        out2[1] = r1[0];
+    }
+}
+
+hipError_t launch_reduce_partials(const double *partial, int count, double *out2,
+                                  hipStream_t s)
+{
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, partial,
+                       count, out2);
+    return hipGetLastError();
+}
+
+// one thread per slot sums its segment in a fixed order
+__global__ void reduce_segments_kernel(const double *partial, int seg, int first,
+                                       int count, double *out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    int s = first + t;
+    const double *p = partial + 2 * (size_t)seg * s;
+    double s0 = 0.0, s1 = 0.0;
+    for (int i = 0; i < seg; i++) {
+        s0 += p[2 * i];
+        s1 += p[2 * i + 1];
+    }
+    out[2 * s] = s0;
+    out[2 * s + 1] = s1;
+}
+
+hipError_t launch_reduce_segments(const double *partial, int seg, int first,
+                                  int count, double *out, hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(reduce_segments_kernel, dim3((count + 63) / 64), dim3(64),
+                       0, s, partial, seg, first, count, out);
+    return hipGetLastError();
+}
+
+__global__ void copy_kernel(const double *src, double *dst, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy(const double *src, double *dst, size_t n, hipStream_t s)
+{
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
+__global__ void nonfinite_kernel(const double *src, size_t n, int *flag)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        if (!isfinite(src[i])) *flag = 1;
+}
+
+hipError_t launch_nonfinite_check(const double *src, size_t n, int *flag,
+                                  hipStream_t s)
+{
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(nonfinite_kernel, dim3(blocks), dim3(256), 0, s, src, n,
+                       flag);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// initial / exact cell averages (T2S4_Parareal.cu:118-140)
+// ---------------------------------------------------------------------------
+struct ExactArgs {
+    int nx, ny, bc, kind, ng;
+    double x_lo, y_lo, dx, dy, Lx, Ly, nu, t;
+    double p[16];
+};
+
+__device__ double rot_gauss(const ExactArgs &e, double x, double y)
+{
+    double x0 = e.p[0], y0 = e.p[1], s = e.p[2], om = e.p[3], cx = e.p[4],
+           cy = e.p[5], t = e.t;
+    double co = cos(om * t), si = sin(om * t);
+    double ax = x - cx, ay = y - cy;
+    double xs = cx + (ax * co + ay * si);
+    double ys = cy + (-ax * si + ay * co);
+    double den = s + 4 * e.nu * t;
+    double ex = xs - x0, ey = ys - y0;
+    return s / den * exp(-(ex * ex + ey * ey) / den);
+}
+
+__device__ double gaussians(const ExactArgs &e, double x, double y)
+{
+    double u = e.p[0], v = e.p[1], nu = e.p[2], t = e.t, acc = 0.0;
+    for (int k = 0; k < e.ng && k < 3; k++) {
+        const double *g = e.p + 3 + 4 * k;
+        double w2 = g[2] * g[2], s2 = w2 + 2.0 * nu * t;
+        double ex = x - g[0] - u * t, ey = y - g[1] - v * t;
+        acc += g[3] * (w2 / s2) * exp(-(ex * ex + ey * ey) / (2.0 * s2));
+    }
+    return acc;
+}
+
+__device__ double cosine_bell(const ExactArgs &e, double x, double y)
+{
+    double ex = x - e.p[0], ey = y - e.p[1];
+    double r = sqrt(ex * ex + ey * ey), R = e.p[2];
+    return r < R ? 0.5 * (1.0 + cos(M_PI * r / R)) : 0.0;
+}
+
+__device__ double point_value(const ExactArgs &e, double x, double y)
+{
+    int img = e.bc == BC_PER ? 1 : 0;
+    double acc = 0.0;
+    for (int ia = -img; ia <= img; ia++)
+        for (int ib = -img; ib <= img; ib++) {
+            double xx = x + ia * e.Lx, yy = y + ib * e.Ly;
+            double v;
+            if (e.kind == CCFV_IC_ROT_GAUSSIAN) v = rot_gauss(e, xx, yy);
+            else if (e.kind == CCFV_IC_GAUSSIANS) v = gaussians(e, xx, yy);
+            else v = cosine_bell(e, xx, yy);
+            acc += v;
+        }
+    return acc;
+}
+
+__global__ void exact_kernel(ExactArgs e, double *out)
+{
+    size_t total = (size_t)e.nx * e.ny;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        int i = (int)(idx / e.ny), j = (int)(idx % e.ny);
+        double x = e.x_lo + (i + 0.5) * e.dx, y = e.y_lo + (j + 0.5) * e.dy;
+        double xm = x - 0.5 * e.dx, xp = x + 0.5 * e.dx;
+        double ym = y - 0.5 * e.dy, yp = y + 0.5 * e.dy;
+        double v;
+        if (e.kind == CCFV_IC_SINSIN) {
+            const double tp = 2.0 * M_PI;
+            double u = e.p[0], w = e.p[1], nu = e.p[2], t = e.t;
+            double ax = (cos(tp * (xm - u * t)) - cos(tp * (xp - u * t))) / (tp * e.dx);
+            double ay = (cos(tp * (ym - w * t)) - cos(tp * (yp - w * t))) / (tp * e.dy);
+            v = ax * ay * exp(-2.0 * tp * tp * nu * t);
+        } else {
+            v = 1.0 / 36.0 *
+                (point_value(e, xm, ym) + point_value(e, xp, ym) +
+                 point_value(e, xm, yp) + point_value(e, xp, yp) +
+                 4.0 * (point_value(e, xm, y) + point_value(e, xp, y) +
+                        point_value(e, x, ym) + point_value(e, x, yp)) +
+                 16.0 * point_value(e, x, y));
+        }
+        out[idx] = v;
+    }
+}
+
+hipError_t launch_fill_exact(const ccfv_problem &p, const ccfv_ic &ic, double t,
+                             double *out, hipStream_t s)
+{
+    ExactArgs e;
+    e.nx = p.nx;
+    e.ny = p.ny;
+    e.bc = p.bc;
+    e.kind = ic.kind;
+    e.ng = ic.n;
+    e.x_lo = p.x_lo;
+    e.y_lo = p.y_lo;
+    e.dx = (p.x_hi - p.x_lo) / p.nx;
+    e.dy = (p.y_hi - p.y_lo) / p.ny;
+    e.Lx = p.x_hi - p.x_lo;
+    e.Ly = p.y_hi - p.y_lo;
+    e.nu = p.nu;
+    e.t = t;
+    for (int k = 0; k < 16; k++) e.p[k] = ic.p[k];
+    size_t total = (size_t)p.nx * p.ny;
+    int blocks = (int)((total + 255) / 256);
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(exact_kernel, dim3(blocks), dim3(256), 0, s, e, out);
+    return hipGetLastError();
+}
+
+}  // namespace ccfv

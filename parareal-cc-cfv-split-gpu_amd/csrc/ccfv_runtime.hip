@@ -4,6 +4,7 @@
 // once Parareal runs, the slice arrays.  All work is queued on the context's
 // own stream (default priority).  Host code never touches field data except
 // through ccfv_set_state / ccfv_get_state.
+#include "ccfv_feet.h"
 #include "ccfv_internal.h"
 
 #include <algorithm>
@@ -83,22 +84,24 @@ Tile env_tile(const char *name, Tile dflt)
 
 bool tile_ok(Tile t, int n, int nl, Dir dir, bool post)
 {
-    if (t.w <= 0 || t.nw <= 0 || t.nw > 8 || t.w % t.nw || nl % t.w) return false;
+    (void)post;
+    // NW <= 4: the sweep kernels are compiled for at most 256 work-items
+    if (t.w <= 0 || t.nw <= 0 || t.nw > 4 || t.w % t.nw || nl % t.w) return false;
+    if (t.w & (t.w - 1)) return false;  // power of two: tiles split by shifts
     if (dir == DIR_X && (t.w & 1)) return false;
-    return sweep_lds_bytes(n, t, post) <= 160 * 1024;
+    return std::max(sweep_lds_bytes_generic(n, t), sweep_lds_bytes_fast(n, t)) <=
+           160 * 1024;
 }
 
-// default tiles; the in-repo autotuner overrides them per problem size
+// default tiles (measured on MI355X at 512^2); ccfv_autotune refines them
 Tile pick_tile(Dir dir, int n, int nl, bool batch, bool post)
 {
-    static const Tile cands_x[] = {{8, 4}, {4, 4}, {4, 2}, {2, 2}, {2, 1}};
-    static const Tile cands_y[] = {{4, 4}, {2, 2}, {1, 1}};
-    const Tile *c = dir == DIR_X ? cands_x : cands_y;
-    int nc = dir == DIR_X ? 5 : 3;
-    int start = 0;
-    if (dir == DIR_X && !batch) start = 1;  // more blocks for one field
-    for (int i = start; i < nc; i++)
-        if (tile_ok(c[i], n, nl, dir, post)) return c[i];
+    static const Tile xb[] = {{8, 4}, {16, 4}, {4, 4}, {4, 2}, {2, 2}, {2, 1}};
+    static const Tile x1[] = {{4, 4}, {2, 2}, {4, 2}, {2, 1}, {8, 4}};
+    static const Tile yb[] = {{2, 1}, {4, 2}, {8, 4}, {1, 1}};
+    static const Tile y1[] = {{1, 1}, {2, 2}, {2, 1}, {4, 4}};
+    const Tile *c = dir == DIR_X ? (batch ? xb : x1) : (batch ? yb : y1);
+    int nc = dir == DIR_X ? (batch ? 6 : 5) : 4;
     for (int i = 0; i < nc; i++)
         if (tile_ok(c[i], n, nl, dir, post)) return c[i];
     return Tile{0, 0};
@@ -190,14 +193,61 @@ int get_table(ccfv_ctx *c, Dir dir, double h, const SolveTable **out)
         T.sf = steps(pf);
         T.sb = steps(pb);
     }
-    std::vector<double> tab(4 * (size_t)n);
+    std::vector<double> tab(4 * (size_t)n + 128, 1.0);
     std::copy(mneg.begin(), mneg.end(), tab.begin());
     std::copy(den.begin(), den.end(), tab.begin() + n);
     std::copy(pneg.begin(), pneg.end(), tab.begin() + 2 * n);
     std::copy(z.begin(), z.end(), tab.begin() + 3 * n);
+    {
+        // chunk products in the order the lanes fold them; 1 on idle lanes
+        const int ch = chunk_for(n), nl = n / ch;
+        for (int l = 0; l < nl; l++) {
+            double f = 1.0, b = 1.0;
+            for (int k = 0; k < ch; k++) f = mneg[l * ch + k] * f;
+            for (int k = ch - 1; k >= 0; k--) b = pneg[l * ch + k] * b;
+            tab[4 * (size_t)n + l] = f;
+            tab[4 * (size_t)n + 64 + l] = b;
+        }
+    }
     HIPCHK(hipMalloc(&T.dev, sizeof(double) * tab.size()));
     HIPCHK(hipMemcpy(T.dev, tab.data(), sizeof(double) * tab.size(),
                      hipMemcpyHostToDevice));
+    // uniform-displacement path: locate face 0 of every line exactly as the
+    // general path does; accept the sweep when every xi is >= 1e-9 from a
+    // cell edge (then every face's floor() agrees with face 0's)
+    const bool uniform = per && (P.vel_kind == CCFV_VEL_SOLID_BODY ||
+                                 P.vel_kind == CCFV_VEL_CONSTANT);
+    if (uniform && !getenv("CCFV_NO_FAST")) {
+        const int nl = dir == DIR_X ? P.ny : P.nx;
+        const double lo = dir == DIR_X ? P.x_lo : P.y_lo;
+        const double olo = dir == DIR_X ? P.y_lo : P.x_lo;
+        const double od = cell_width(P, dir == DIR_X ? DIR_Y : DIR_X);
+        const double tau = dir == DIR_X ? 0.5 * h : h;
+        const double inv12d = 1.0 / (12.0 * d);
+        VelParams v{P.vel_kind, P.vel[0], P.vel[1], P.vel[2]};
+        std::vector<double> ld(6 * (size_t)nl);
+        bool ok = true;
+        for (int L = 0; L < nl && ok; L++) {
+            double o0 = corner(olo, od, L), o1 = corner(olo, od, L + 1);
+            double xb = dir == DIR_X
+                            ? face_foot<0>(v, lo, d, tau, 0.0, 0.0, 0, o0, o1)
+                            : face_foot<1>(v, lo, d, tau, 0.0, 0.0, 0, o0, o1);
+            double xi;
+            int m = cell_of(lo, d, n, xb, xi);
+            if (!(xi > 1e-9 && xi < 1.0 - 1e-9) || std::abs(m) > 2 * n) {
+                ok = false;
+                break;
+            }
+            ld[L] = (double)m;
+            fir_weights(xi, T.coef, inv12d, &ld[nl + 5 * (size_t)L]);
+        }
+        if (ok) {
+            HIPCHK(hipMalloc(&T.line_dev, sizeof(double) * ld.size()));
+            HIPCHK(hipMemcpy(T.line_dev, ld.data(), sizeof(double) * ld.size(),
+                             hipMemcpyHostToDevice));
+            T.fast_ok = true;
+        }
+    }
     auto res = c->tables.emplace(key, T);
     *out = &res.first->second;
     return CCFV_OK;
@@ -230,8 +280,6 @@ SweepArgs make_args(ccfv_ctx *c, Dir dir, int substep, const SolveTable &T,
     a.d = cell_width(P, dir);
     a.od = cell_width(P, dir == DIR_X ? DIR_Y : DIR_X);
     a.quirk = (P.flags & CCFV_FLAG_DERIV_GHOST_QUIRK) ? 1 : 0;
-    a.uniform = P.bc == CCFV_BC_PERIODIC && (P.vel_kind == CCFV_VEL_SOLID_BODY ||
-                                             P.vel_kind == CCFV_VEL_CONSTANT);
     a.sf = T.sf;
     a.sb = T.sb;
     a.coef = T.coef;
@@ -240,6 +288,8 @@ SweepArgs make_args(ccfv_ctx *c, Dir dir, int substep, const SolveTable &T,
     a.vl = T.vl;
     a.smden = T.smden;
     a.tab = T.dev;
+    a.lineS = T.line_dev;
+    a.lineK = T.line_dev ? T.line_dev + a.nl : nullptr;
     a.vel_kind = P.vel_kind;
     a.v0 = P.vel[0];
     a.v1 = P.vel[1];
@@ -284,11 +334,18 @@ void drain_records(ccfv_ctx *c)
 }
 
 // launch one sweep, timed when profiling is on
+hipError_t run_sweep(ccfv_ctx *c, Dir dir, bool post, Epi epi, const SweepArgs &a,
+                     int nf, Tile t)
+{
+    if (nf <= 0) return hipSuccess;
+    if (a.lineS) return launch_sweep_fast(dir, epi, a, nf, t, c->stream);
+    return launch_sweep_generic(dir, c->P.bc, post, epi, a, nf, t, c->stream);
+}
+
 hipError_t timed_sweep(ccfv_ctx *c, int kind, double bytes, Dir dir, bool post,
                        Epi epi, const SweepArgs &a, int nf, Tile t)
 {
-    if (!c->prof)
-        return launch_sweep(dir, c->P.bc, post, epi, a, nf, t, c->stream);
+    if (!c->prof) return run_sweep(c, dir, post, epi, a, nf, t);
     if (c->recs.size() > 4096) {
         hipError_t e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) return e;
@@ -296,7 +353,7 @@ hipError_t timed_sweep(ccfv_ctx *c, int kind, double bytes, Dir dir, bool post,
     }
     ccfv_ctx::Rec r{kind, bytes, pool_event(c), pool_event(c)};
     if (r.a) (void)hipEventRecord(r.a, c->stream);
-    hipError_t e = launch_sweep(dir, c->P.bc, post, epi, a, nf, t, c->stream);
+    hipError_t e = run_sweep(c, dir, post, epi, a, nf, t);
     if (r.b) (void)hipEventRecord(r.b, c->stream);
     c->recs.push_back(r);
     return e;
@@ -410,7 +467,8 @@ void free_para(ccfv_ctx *c)
 
 int ensure_para(ccfv_ctx *c, int ns)
 {
-    if (c->para_ns == ns) return CCFV_OK;
+    if (c->para_ns == ns && c->chain_blocks == c->P.ny / c->tile[DIR_X][0].w)
+        return CCFV_OK;
     free_para(c);
     const size_t fb = sizeof(double) * c->ncell;
     c->chain_blocks = c->P.ny / c->tile[DIR_X][0].w;
@@ -547,7 +605,10 @@ void ccfv_destroy(ccfv_ctx *c)
     free_para(c);
     drain_records(c);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-    for (auto &kv : c->tables) (void)hipFree(kv.second.dev);
+    for (auto &kv : c->tables) {
+        (void)hipFree(kv.second.dev);
+        (void)hipFree(kv.second.line_dev);
+    }
     hipFree(c->state);
     hipFree(c->d_flag);
     if (c->h_flag) hipHostFree(c->h_flag);
@@ -588,6 +649,13 @@ int ccfv_get_state_device(ccfv_ctx *c, double *dev)
     if (!c || !dev) return fail(CCFV_EINVAL, "NULL argument");
     HIPCHK(launch_copy(c->state, dev, c->ncell, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    return CCFV_OK;
+}
+
+int ccfv_state_device_ptr(ccfv_ctx *c, void **dev)
+{
+    if (!c || !dev) return fail(CCFV_EINVAL, "NULL argument");
+    *dev = c->state;
     return CCFV_OK;
 }
 
@@ -673,6 +741,118 @@ int ccfv_parareal_correction(ccfv_ctx *c, const double *g_new,
     if (blocks < 1) blocks = 1;
     HIPCHK(launch_correction(g_new, g_old, f, u, n, nullptr, blocks, c->stream));
     return CCFV_OK;
+}
+
+int ccfv_get_tiles(ccfv_ctx *c, int32_t t[8])
+{
+    if (!c || !t) return fail(CCFV_EINVAL, "NULL argument");
+    const Tile v[4] = {c->tile[DIR_X][0], c->tile[DIR_X][1], c->tile[DIR_Y][0],
+                       c->tile[DIR_Y][1]};
+    for (int i = 0; i < 4; i++) {
+        t[2 * i] = v[i].w;
+        t[2 * i + 1] = v[i].nw;
+    }
+    return CCFV_OK;
+}
+
+int ccfv_set_tiles(ccfv_ctx *c, const int32_t t[8])
+{
+    if (!c || !t) return fail(CCFV_EINVAL, "NULL argument");
+    const bool post = post_flag(c);
+    Tile v[4];
+    for (int i = 0; i < 4; i++) {
+        v[i] = Tile{t[2 * i], t[2 * i + 1]};
+        Dir d = i < 2 ? DIR_X : DIR_Y;
+        int n = d == DIR_X ? c->P.nx : c->P.ny, nl = d == DIR_X ? c->P.ny : c->P.nx;
+        if (!tile_ok(v[i], n, nl, d, post && d == DIR_X))
+            return fail(CCFV_EINVAL, "invalid tile shape for this grid");
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->tile[DIR_X][0] = v[0];
+    c->tile[DIR_X][1] = v[1];
+    c->tile[DIR_Y][0] = v[2];
+    c->tile[DIR_Y][1] = v[3];
+    return CCFV_OK;
+}
+
+int ccfv_autotune(ccfv_ctx *c, int32_t batch_fields, double h, int32_t out[8])
+{
+    if (!c) return fail(CCFV_EINVAL, "NULL context");
+    if (batch_fields < 1 || batch_fields > 4096)
+        return fail(CCFV_EINVAL, "batch_fields must be in [1, 4096]");
+    if (!(h > 0.0)) return fail(CCFV_EINVAL, "h must be > 0");
+    const SolveTable *tx, *ty;
+    int rc = get_table(c, DIR_X, h, &tx);
+    if (rc) return rc;
+    rc = get_table(c, DIR_Y, h, &ty);
+    if (rc) return rc;
+    const bool post = post_flag(c);
+    const size_t N = c->ncell;
+    double *scratch = nullptr;
+    HIPCHK(hipMalloc(&scratch, sizeof(double) * N * (size_t)batch_fields));
+    for (int f = 0; f < batch_fields; f++) {
+        hipError_t e = launch_copy(c->state, scratch + f * N, N, c->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(scratch);
+            return hip_fail(e, "autotune copy");
+        }
+    }
+    static const int ws[] = {1, 2, 4, 8, 16, 32};
+    static const int nws[] = {1, 2, 4};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    int status = CCFV_OK;
+    for (int d = 0; d < 2 && !status; d++) {
+        const Dir dir = (Dir)d;
+        const int n = dir == DIR_X ? c->P.nx : c->P.ny;
+        const int nl = dir == DIR_X ? c->P.ny : c->P.nx;
+        for (int b = 0; b < 2 && !status; b++) {
+            const int nf = b ? batch_fields : 1;
+            const int reps = b ? 3 : 20;
+            SweepArgs a = make_args(c, dir, dir == DIR_X ? 1 : 2,
+                                    dir == DIR_X ? *tx : *ty, h, 0.0, 0.0, 0, 0.0);
+            a.in = scratch;
+            a.out = scratch;
+            a.fstride = (long long)N;
+            float best = 1e30f;
+            Tile bt = c->tile[d][b];
+            for (int w : ws)
+                for (int nw : nws) {
+                    Tile t{w, nw};
+                    if (!tile_ok(t, n, nl, dir, post && dir == DIR_X)) continue;
+                    float tbest = 1e30f;
+                    for (int trial = 0; trial < 2; trial++) {
+                        hipError_t e = run_sweep(c, dir, false, EPI_NONE, a, nf, t);
+                        if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+                        for (int r = 0; r < reps && e == hipSuccess; r++)
+                            e = run_sweep(c, dir, false, EPI_NONE, a, nf, t);
+                        if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+                        if (e == hipSuccess) e = hipEventSynchronize(e1);
+                        float ms = 0.f;
+                        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                        if (e != hipSuccess) {
+                            status = hip_fail(e, "autotune sweep");
+                            break;
+                        }
+                        tbest = std::min(tbest, ms / reps);
+                    }
+                    if (status) break;
+                    if (tbest < best) {
+                        best = tbest;
+                        bt = t;
+                    }
+                }
+            c->tile[d][b] = bt;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(scratch);
+    if (status) return status;
+    if (out) ccfv_get_tiles(c, out);
+    return check_flag(c);
 }
 
 int ccfv_profile_enable(ccfv_ctx *c, int32_t enable)

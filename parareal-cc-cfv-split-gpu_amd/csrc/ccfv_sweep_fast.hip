@@ -1,0 +1,267 @@
+// ccfv_sweep_fast.hip — line sweeps for a uniform foot offset per line.
+//
+// When the velocity component along a line does not depend on the
+// along-line coordinate (solid-body rotation, constant velocity) and the
+// boundary is periodic, every face foot of a line sits at the same offset xi
+// in cell q+S.  The host locates face 0 of every line with the same
+// contraction-free arithmetic as the general path and accepts the line only
+// when xi is at least 1e-9 away from a cell edge, so each face's floor()
+// agrees; it stores S and the five filter weights kappa(xi) per line.  The
+// PPM antiderivative, remap and foot derivative (T2S4_Parareal.cu:260-547)
+// then collapse to
+//     b_i = C_{i+S} + K_{i+S+1} - K_{i+S},   K_m = sum_s kappa_s C_{m-2+s},
+// followed by the compact operator and the tridiagonal solve.
+#include "ccfv_line.h"
+
+namespace ccfv {
+
+// Thomas factors of this lane's fixed cell positions.  For the symmetric
+// matrix -p_e == mneg_e for 0 < e < n-1, so one array serves both passes
+// (pneg_0 is a scalar, pneg_{n-1} = 0).  Small chunks keep them in registers
+// for every line a wave sweeps; large chunks read the global table.
+template <int CH, bool REG>
+struct LaneTab;
+
+template <int CH>
+struct LaneTab<CH, true> {
+    double m[CH], den[CH], z[CH];
+    __device__ __forceinline__ void load(const SweepArgs &a, int base, bool active)
+    {
+        const int n = a.n;
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            int e = active ? base + k : 0;
+            m[k] = a.tab[e];
+            den[k] = a.tab[n + e];
+            z[k] = a.tab[3 * n + e];
+        }
+    }
+    __device__ __forceinline__ double M(int k, int) const { return m[k]; }
+    __device__ __forceinline__ double D(int k, int) const { return den[k]; }
+    __device__ __forceinline__ double Z(int k, int) const { return z[k]; }
+};
+
+template <int CH>
+struct LaneTab<CH, false> {
+    const double *t;
+    int n;
+    __device__ __forceinline__ void load(const SweepArgs &a, int, bool)
+    {
+        t = a.tab;
+        n = a.n;
+    }
+    __device__ __forceinline__ double M(int, int e) const { return t[e]; }
+    __device__ __forceinline__ double D(int, int e) const { return t[n + e]; }
+    __device__ __forceinline__ double Z(int, int e) const { return t[3 * n + e]; }
+};
+
+template <int CH, bool REG>
+__device__ __forceinline__ void fast_line(const SweepArgs &a, double *cs,
+                                          int line, int lane,
+                                          const LaneTab<CH, REG> &tb, double mfl,
+                                          double mbl, double pneg0)
+{
+    const int n = a.n, NL = n / CH;
+    const bool active = lane < NL;
+    const int base = lane * CH;
+    const int S = (int)a.lineS[line];
+    const double *kp = a.lineK + 5 * (size_t)line;
+    const double k0 = kp[0], k1 = kp[1], k2 = kp[2], k3 = kp[3], k4 = kp[4];
+    double b[CH];
+    if (active) {
+        double cw[CH + 5];
+        int k = base + S - 2;
+        k %= n;
+        k = k < 0 ? k + n : k;
+#pragma unroll
+        for (int t = 0; t < CH + 5; t++) {
+            cw[t] = cs[pidx(k)];
+            k = k + 1 == n ? 0 : k + 1;
+        }
+        double Kp = k0 * cw[0] + k1 * cw[1] + k2 * cw[2] + k3 * cw[3] + k4 * cw[4];
+#pragma unroll
+        for (int q = 0; q < CH; q++) {
+            double Kn = k0 * cw[q + 1] + k1 * cw[q + 2] + k2 * cw[q + 3] +
+                        k3 * cw[q + 4] + k4 * cw[q + 5];
+            b[q] = cw[q + 2] + (Kn - Kp);
+            Kp = Kn;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < CH; q++) b[q] = 0.0;
+    }
+
+    // compact operator, periodic (T2S4_Parareal.cu:616-654 with wrap-around)
+    double bprev = __shfl_up(b[CH - 1], 1);
+    double bnext = __shfl_down(b[0], 1);
+    double wrap_last = __shfl(b[CH - 1], NL - 1);
+    double wrap_first = __shfl(b[0], 0);
+    if (lane == 0) bprev = wrap_last;
+    if (lane == NL - 1) bnext = wrap_first;
+    double r[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        double bn = k + 1 < CH ? b[k + 1] : bnext;
+        double bp = k > 0 ? b[k - 1] : bprev;
+        r[k] = (bn + 10.0 * b[k] + bp) * (1.0 / 12.0);
+    }
+
+    // forward pass q_e = r_e den_e + mneg_e q_{e-1} (mneg_0 = 0)
+    double M = mfl, Q = 0.0;
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            double c = r[k] * tb.D(k, base + k);
+            r[k] = c;
+            Q = c + tb.M(k, base + k) * Q;
+        }
+    }
+    for (int st = 0, off = 1; st < a.sf; st++, off <<= 1) {
+        double Mp = __shfl_up(M, off), Qp = __shfl_up(Q, off);
+        if (lane >= off) {
+            Q = M * Qp + Q;
+            M = M * Mp;
+        }
+    }
+    double carry = __shfl_up(Q, 1);
+    if (lane == 0) carry = 0.0;
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            carry = r[k] + tb.M(k, base + k) * carry;
+            r[k] = carry;
+        }
+    }
+
+    // back substitution x_e = q_e + pneg_e x_{e+1}
+    M = mbl;
+    Q = 0.0;
+    if (active) {
+#pragma unroll
+        for (int k = CH - 1; k >= 0; k--) {
+            int e = base + k;
+            double pn = e == 0 ? pneg0 : (e == n - 1 ? 0.0 : tb.M(k, e));
+            Q = r[k] + pn * Q;
+        }
+    }
+    for (int st = 0, off = 1; st < a.sb; st++, off <<= 1) {
+        double Mn = __shfl_down(M, off), Qn = __shfl_down(Q, off);
+        if (lane + off < 64) {
+            Q = M * Qn + Q;
+            M = M * Mn;
+        }
+    }
+    carry = __shfl_down(Q, 1);
+    if (lane == 63) carry = 0.0;
+    if (active) {
+#pragma unroll
+        for (int k = CH - 1; k >= 0; k--) {
+            int e = base + k;
+            double pn = e == 0 ? pneg0 : (e == n - 1 ? 0.0 : tb.M(k, e));
+            carry = r[k] + pn * carry;
+            r[k] = carry;
+        }
+    }
+
+    // Sherman-Morrison: x = y - ((y_0 + vl y_{n-1}) / (1 + v.z)) z
+    double y0 = __shfl(r[0], 0);
+    double yn = __shfl(r[CH - 1], NL - 1);
+    double fac = (y0 + a.vl * yn) / a.smden;
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) cs[pidx(base + k)] = r[k] - fac * tb.Z(k, base + k);
+    }
+}
+
+template <int CH, int EPI>
+__global__ __launch_bounds__(256) void fast_x_kernel(SweepArgs a, int W, int NW)
+{
+    extern __shared__ double lds[];
+    constexpr bool REG = CH <= 8;
+    const int n = a.n, LP = line_pitch(n);
+    const size_t foff = (size_t)blockIdx.y * (size_t)a.fstride;
+    const int j0 = blockIdx.x * W;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    LaneTab<CH, REG> tb;
+    tb.load(a, lane * CH, lane < n / CH);
+    const double mfl = a.tab[4 * n + lane], mbl = a.tab[4 * n + 64 + lane];
+    const double pneg0 = a.tab[2 * n];
+    load_tile_x(a.in + foff, a.ld, n, j0, W, lds, LP);
+    __syncthreads();
+    for (int l = wave; l < W; l += NW)
+        fast_line<CH, REG>(a, lds + l * LP, j0 + l, lane, tb, mfl, mbl, pneg0);
+    __syncthreads();
+    if (EPI == EPI_NONE) store_tile_x(a.out + foff, a.ld, n, j0, W, lds, LP);
+    else epilogue_x<EPI>(a, n, j0, W, lds, LP);
+}
+
+template <int CH>
+__global__ __launch_bounds__(256) void fast_y_kernel(SweepArgs a, int W, int NW)
+{
+    extern __shared__ double lds[];
+    constexpr bool REG = CH <= 8;
+    const int n = a.n, LP = line_pitch(n);
+    const size_t foff = (size_t)blockIdx.y * (size_t)a.fstride;
+    const int i0 = blockIdx.x * W;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    LaneTab<CH, REG> tb;
+    tb.load(a, lane * CH, lane < n / CH);
+    const double mfl = a.tab[4 * n + lane], mbl = a.tab[4 * n + 64 + lane];
+    const double pneg0 = a.tab[2 * n];
+    load_tile_y(a.in + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
+    __syncthreads();
+    for (int l = wave; l < W; l += NW)
+        fast_line<CH, REG>(a, lds + l * LP, i0 + l, lane, tb, mfl, mbl, pneg0);
+    __syncthreads();
+    store_tile_y(a.out + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
+}
+
+size_t sweep_lds_bytes_fast(int n, Tile tile)
+{
+    return sizeof(double) * (size_t)line_pitch(n) * (size_t)tile.w;
+}
+
+template <typename K>
+static hipError_t launch(K k, dim3 grid, dim3 block, size_t lds, hipStream_t s,
+                         const SweepArgs &a, Tile t)
+{
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute(
+            (const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL(k, grid, block, lds, s, a, t.w, t.nw);
+    return hipGetLastError();
+}
+
+template <int CH>
+static hipError_t dispatch(Dir dir, Epi epi, const SweepArgs &a, int nf, Tile t,
+                           hipStream_t s)
+{
+    dim3 grid(a.nl / t.w, nf), block(64 * t.nw);
+    size_t lds = sweep_lds_bytes_fast(a.n, t);
+    if (dir == DIR_Y) return launch(fast_y_kernel<CH>, grid, block, lds, s, a, t);
+    if (epi == EPI_NONE) return launch(fast_x_kernel<CH, EPI_NONE>, grid, block, lds, s, a, t);
+    if (epi == EPI_INIT) return launch(fast_x_kernel<CH, EPI_INIT>, grid, block, lds, s, a, t);
+    return launch(fast_x_kernel<CH, EPI_CORRECT>, grid, block, lds, s, a, t);
+}
+
+hipError_t launch_sweep_fast(Dir dir, Epi epi, const SweepArgs &a, int nfields,
+                             Tile tile, hipStream_t s)
+{
+    switch (chunk_for(a.n)) {
+    case 1: return dispatch<1>(dir, epi, a, nfields, tile, s);
+    case 2: return dispatch<2>(dir, epi, a, nfields, tile, s);
+    case 4: return dispatch<4>(dir, epi, a, nfields, tile, s);
+    case 8: return dispatch<8>(dir, epi, a, nfields, tile, s);
+    case 16: return dispatch<16>(dir, epi, a, nfields, tile, s);
+    case 32: return dispatch<32>(dir, epi, a, nfields, tile, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace ccfv

@@ -1,0 +1,162 @@
+// ccfv_sweep_generic.hip — general line sweeps: every face foot traced and
+// located on its own (closed reference box, time-dependent or non-uniform
+// velocity, and any line the uniform path cannot take).
+//
+// One launch performs one sub-step of the Strang step (T2S4_Parareal.cu:
+// 825-870, :873-917 or :920-963) for a batch of fields.  A block stages W
+// lines in LDS; each wavefront sweeps whole lines; the tile is written back
+// in place (every sweep is line-local).
+#include "ccfv_line.h"
+
+namespace ccfv {
+
+// cs: the line in LDS (pidx layout), overwritten with the result.
+// aux: a line-sized LDS buffer owned by this wave.
+template <int DIR, int BC, int CH, bool POST>
+__device__ void process_line(const SweepArgs &a, double *cs, double *aux,
+                             int line, int lane)
+{
+    const int n = a.n;
+    const int NL = n / CH;
+    const bool active = lane < NL;
+    const int base = lane * CH;
+    const double inv12d = 1.0 / (12.0 * a.d);
+    const double o0 = corner(a.olo, a.od, line), o1 = corner(a.olo, a.od, line + 1);
+    double g[4] = {0.0, 0.0, 0.0, 0.0};
+    if (BC == BC_REF) ghosts(cs, n, g);
+
+    double b[CH];
+    if (active) rhs_general<DIR, BC>(a, cs, g, aux, base, CH, o0, o1, inv12d, POST);
+    if (POST) {
+        // the reference's third sub-step takes the derivative of the remapped
+        // line (T2S4_Parareal.cu:929-932)
+        __syncthreads();
+        double gs[4] = {0.0, 0.0, 0.0, 0.0};
+        if (BC == BC_REF) ghosts(aux, n, gs);
+        if (active) {
+            double dl = deriv_ref<BC>(a, aux, gs, face_foot<DIR>(a, base, o0, o1),
+                                      inv12d);
+#pragma unroll
+            for (int k = 0; k < CH; k++) {
+                double dr = deriv_ref<BC>(a, aux, gs,
+                                          face_foot<DIR>(a, base + k + 1, o0, o1),
+                                          inv12d);
+                b[k] = aux[pidx(base + k)] + a.coef * (dr - dl);
+                dl = dr;
+            }
+        }
+        __syncthreads();
+    } else if (active) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) b[k] = aux[pidx(base + k)];
+    }
+    if (!active) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) b[k] = 0.0;
+    }
+    const double *t = a.tab;
+    solve_tail<BC, CH, false>(a, b, lane, active, t, t + n, t + 2 * n, t + 3 * n,
+                              t + 4 * n, t + 4 * n + 64, cs);
+}
+
+template <int BC, int CH, bool POST, int EPI>
+__global__ __launch_bounds__(512) void sweep_x_kernel(SweepArgs a, int W, int NW)
+{
+    extern __shared__ double lds[];
+    const int n = a.n, LP = line_pitch(n);
+    const size_t foff = (size_t)blockIdx.y * (size_t)a.fstride;
+    const int j0 = blockIdx.x * W;
+    load_tile_x(a.in + foff, a.ld, n, j0, W, lds, LP);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double *aux = lds + (W + wave) * LP;
+    for (int l = wave; l < W; l += NW)
+        process_line<DIR_X, BC, CH, POST>(a, lds + l * LP, aux, j0 + l, lane);
+    __syncthreads();
+    if (EPI == EPI_NONE) store_tile_x(a.out + foff, a.ld, n, j0, W, lds, LP);
+    else epilogue_x<EPI>(a, n, j0, W, lds, LP);
+}
+
+template <int BC, int CH>
+__global__ __launch_bounds__(512) void sweep_y_kernel(SweepArgs a, int W, int NW)
+{
+    extern __shared__ double lds[];
+    const int n = a.n, LP = line_pitch(n);
+    const size_t foff = (size_t)blockIdx.y * (size_t)a.fstride;
+    const int i0 = blockIdx.x * W;
+    load_tile_y(a.in + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double *aux = lds + (W + wave) * LP;
+    for (int l = wave; l < W; l += NW)
+        process_line<DIR_Y, BC, CH, false>(a, lds + l * LP, aux, i0 + l, lane);
+    __syncthreads();
+    store_tile_y(a.out + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
+}
+
+size_t sweep_lds_bytes_generic(int n, Tile tile)
+{
+    // W tile lines + one auxiliary line per wave
+    return sizeof(double) * (size_t)line_pitch(n) * (size_t)(tile.w + tile.nw);
+}
+
+template <typename K>
+static hipError_t launch(K k, dim3 grid, dim3 block, size_t lds, hipStream_t s,
+                         const SweepArgs &a, Tile t)
+{
+    static bool attr = false;  // one attribute call per instantiation
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute(
+            (const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL(k, grid, block, lds, s, a, t.w, t.nw);
+    return hipGetLastError();
+}
+
+template <int BC, int CH>
+static hipError_t dispatch(Dir dir, bool post, Epi epi, const SweepArgs &a,
+                           int nf, Tile t, hipStream_t s)
+{
+    dim3 grid(a.nl / t.w, nf), block(64 * t.nw);
+    size_t lds = sweep_lds_bytes_generic(a.n, t);
+    if (dir == DIR_Y) return launch(sweep_y_kernel<BC, CH>, grid, block, lds, s, a, t);
+    if (BC == BC_REF && post) {
+        if (epi == EPI_NONE)
+            return launch(sweep_x_kernel<BC, CH, true, EPI_NONE>, grid, block, lds, s, a, t);
+        if (epi == EPI_INIT)
+            return launch(sweep_x_kernel<BC, CH, true, EPI_INIT>, grid, block, lds, s, a, t);
+        return launch(sweep_x_kernel<BC, CH, true, EPI_CORRECT>, grid, block, lds, s, a, t);
+    }
+    if (epi == EPI_NONE)
+        return launch(sweep_x_kernel<BC, CH, false, EPI_NONE>, grid, block, lds, s, a, t);
+    if (epi == EPI_INIT)
+        return launch(sweep_x_kernel<BC, CH, false, EPI_INIT>, grid, block, lds, s, a, t);
+    return launch(sweep_x_kernel<BC, CH, false, EPI_CORRECT>, grid, block, lds, s, a, t);
+}
+
+template <int BC>
+static hipError_t dispatch_bc(Dir dir, bool post, Epi epi, const SweepArgs &a,
+                              int nf, Tile t, hipStream_t s)
+{
+    switch (chunk_for(a.n)) {
+    case 1: return dispatch<BC, 1>(dir, post, epi, a, nf, t, s);
+    case 2: return dispatch<BC, 2>(dir, post, epi, a, nf, t, s);
+    case 4: return dispatch<BC, 4>(dir, post, epi, a, nf, t, s);
+    case 8: return dispatch<BC, 8>(dir, post, epi, a, nf, t, s);
+    case 16: return dispatch<BC, 16>(dir, post, epi, a, nf, t, s);
+    case 32: return dispatch<BC, 32>(dir, post, epi, a, nf, t, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_sweep_generic(Dir dir, int bc, bool post, Epi epi,
+                                const SweepArgs &a, int nfields, Tile tile,
+                                hipStream_t s)
+{
+    if (bc == BC_PER) return dispatch_bc<BC_PER>(dir, false, epi, a, nfields, tile, s);
+    return dispatch_bc<BC_REF>(dir, post, epi, a, nfields, tile, s);
+}
+
+}  // namespace ccfv
