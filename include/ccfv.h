@@ -192,6 +192,27 @@ int ccfv_parareal_solve(ccfv_ctx *ctx, double t0, double t1, int32_t nt_fine,
                         int32_t n_slices, int32_t kmax, double tol,
                         ccfv_parareal_report *rep);
 
+/* Parareal schedule options (no counterpart in the reference, which runs
+ * every slice serially, :1302-1330):
+ *   CCFV_OPT_PIPELINE   1 (default): the fine propagators of iteration k+1
+ *                       run on a second stream and overlap the coarse chain
+ *                       of iteration k group by group; 0: one stream, fine
+ *                       batch then chain.  Both give identical results.
+ *   CCFV_OPT_PIPE_GROUP slices per group of the pipelined schedule (32).
+ *   CCFV_OPT_GRAPHS     1: each pipelined phase (coarse chain of iteration k
+ *                       with the fine propagators of k+1) is captured once
+ *                       as a hipGraph and replayed by later solves with the
+ *                       same times, slices and shapes (0, default: issued
+ *                       directly, which keeps the overlap across phases).
+ *   CCFV_OPT_PLAIN_GRAPHS 1: ccfv_plain_solve captures the whole solve as a
+ *                       hipGraph and replays it for identical solves. */
+#define CCFV_OPT_PIPELINE     1
+#define CCFV_OPT_PIPE_GROUP   2
+#define CCFV_OPT_GRAPHS       3
+#define CCFV_OPT_PLAIN_GRAPHS 4
+int ccfv_set_option(ccfv_ctx *ctx, int32_t key, int64_t value);
+int ccfv_get_option(ccfv_ctx *ctx, int32_t key, int64_t *value);
+
 /* U = G_new + (F - G_old) over n doubles, device pointers, on the context's
  * stream (parareal_correction_kernel, :771-782). */
 int ccfv_parareal_correction(ccfv_ctx *ctx, const double *g_new,

@@ -66,6 +66,9 @@ _SIGS = {
     "ccfv_parareal_correction": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4
                                  + [ctypes.c_size_t]),
     "ccfv_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "ccfv_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64]),
+    "ccfv_get_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
+                                       ctypes.POINTER(ctypes.c_int64)]),
     "ccfv_get_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     "ccfv_set_tiles": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     "ccfv_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double,
@@ -219,6 +222,14 @@ class Context:
 
     def sync(self):
         _check(load().ccfv_sync(self._h), "ccfv_sync")
+
+    def set_option(self, key: int, value: int):
+        _check(load().ccfv_set_option(self._h, key, value), "ccfv_set_option")
+
+    def get_option(self, key: int) -> int:
+        v = ctypes.c_int64()
+        _check(load().ccfv_get_option(self._h, key, ctypes.byref(v)), "ccfv_get_option")
+        return v.value
 
     def get_tiles(self) -> list:
         t = (ctypes.c_int32 * 8)()

@@ -42,6 +42,11 @@ IC_COSINE_BELL = 3
 
 MAX_ITERS = 64
 
+OPT_PIPELINE = 1
+OPT_PIPE_GROUP = 2
+OPT_GRAPHS = 3
+OPT_PLAIN_GRAPHS = 4
+
 
 class Problem(ctypes.Structure):
     """ccfv_problem."""

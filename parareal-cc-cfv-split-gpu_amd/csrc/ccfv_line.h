@@ -440,6 +440,16 @@ __device__ __forceinline__ void solve_tail(const SweepArgs &a, double (&b)[CH],
 // W is a power of two (enforced by the host), so idx splits by shifts.
 __device__ __forceinline__ int log2i(int v) { return 31 - __builtin_clz(v); }
 
+// Column tile of an x-sweep block.  Workgroups are dealt round-robin to the
+// eight XCDs, each with its own L2; give every XCD a contiguous range of
+// column tiles so that the 128-byte lines its tiles touch are shared there
+// instead of being fetched by several XCDs.
+__device__ __forceinline__ int xcd_tile(int bx, int gx)
+{
+    if (gx & 7) return bx;
+    return (bx & 7) * (gx >> 3) + (bx >> 3);
+}
+
 __device__ __forceinline__ void load_tile_x(const double *in, int ld, int n,
                                             int j0, int W, double *lds, int LP)
 {

@@ -45,7 +45,9 @@ int hip_fail(hipError_t e, const char *what)
 struct ccfv_ctx {
     ccfv_problem P{};
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // the context's stream (coarse chain)
+    hipStream_t stream2 = nullptr;  // fine propagators when pipelined
+    hipStream_t cur = nullptr;      // stream the next sweep launches go to
     size_t ncell = 0;
     double *state = nullptr;
     int *d_flag = nullptr;   // crossed-feet / non-finite flag
@@ -59,6 +61,17 @@ struct ccfv_ctx {
     double *partial = nullptr, *slot_sums = nullptr, *h_sums = nullptr;
     int chain_blocks = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // pipelined Parareal: fine propagators of iteration k+1 overlap the
+    // coarse chain of iteration k, group by group
+    bool pipeline = true;
+    int pipe_group = 32;
+    // captured Parareal phases, keyed by plan; cleared when buffers or
+    // launch shapes change
+    bool graphs = false;
+    bool plain_graphs = false;
+    int tiles_version = 0;
+    std::map<std::string, hipGraphExec_t> gcache;
+    std::vector<hipEvent_t> sync_ev;  // cross-stream events (no timing)
     // kernel timing
     bool prof = false;
     struct Rec {
@@ -97,9 +110,9 @@ bool tile_ok(Tile t, int n, int nl, Dir dir, bool post)
 Tile pick_tile(Dir dir, int n, int nl, bool batch, bool post)
 {
     static const Tile xb[] = {{8, 4}, {16, 4}, {4, 4}, {4, 2}, {2, 2}, {2, 1}};
-    static const Tile x1[] = {{4, 4}, {2, 2}, {4, 2}, {2, 1}, {8, 4}};
+    static const Tile x1[] = {{2, 2}, {4, 4}, {4, 2}, {2, 1}, {8, 4}};
     static const Tile yb[] = {{2, 1}, {4, 2}, {8, 4}, {1, 1}};
-    static const Tile y1[] = {{1, 1}, {2, 2}, {2, 1}, {4, 4}};
+    static const Tile y1[] = {{2, 2}, {1, 1}, {2, 1}, {4, 4}};
     const Tile *c = dir == DIR_X ? (batch ? xb : x1) : (batch ? yb : y1);
     int nc = dir == DIR_X ? (batch ? 6 : 5) : 4;
     for (int i = 0; i < nc; i++)
@@ -338,8 +351,8 @@ hipError_t run_sweep(ccfv_ctx *c, Dir dir, bool post, Epi epi, const SweepArgs &
                      int nf, Tile t)
 {
     if (nf <= 0) return hipSuccess;
-    if (a.lineS) return launch_sweep_fast(dir, epi, a, nf, t, c->stream);
-    return launch_sweep_generic(dir, c->P.bc, post, epi, a, nf, t, c->stream);
+    if (a.lineS) return launch_sweep_fast(dir, epi, a, nf, t, c->cur);
+    return launch_sweep_generic(dir, c->P.bc, post, epi, a, nf, t, c->cur);
 }
 
 hipError_t timed_sweep(ccfv_ctx *c, int kind, double bytes, Dir dir, bool post,
@@ -348,13 +361,14 @@ hipError_t timed_sweep(ccfv_ctx *c, int kind, double bytes, Dir dir, bool post,
     if (!c->prof) return run_sweep(c, dir, post, epi, a, nf, t);
     if (c->recs.size() > 4096) {
         hipError_t e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream2);
         if (e != hipSuccess) return e;
         drain_records(c);
     }
     ccfv_ctx::Rec r{kind, bytes, pool_event(c), pool_event(c)};
-    if (r.a) (void)hipEventRecord(r.a, c->stream);
+    if (r.a) (void)hipEventRecord(r.a, c->cur);
     hipError_t e = run_sweep(c, dir, post, epi, a, nf, t);
-    if (r.b) (void)hipEventRecord(r.b, c->stream);
+    if (r.b) (void)hipEventRecord(r.b, c->cur);
     c->recs.push_back(r);
     return e;
 }
@@ -452,8 +466,15 @@ int validate_problem(const ccfv_problem *p)
     return CCFV_OK;
 }
 
+void clear_graphs(ccfv_ctx *c)
+{
+    for (auto &kv : c->gcache) (void)hipGraphExecDestroy(kv.second);
+    c->gcache.clear();
+}
+
 void free_para(ccfv_ctx *c)
 {
+    clear_graphs(c);
     hipFree(c->U);
     hipFree(c->G);
     hipFree(c->F);
@@ -479,10 +500,10 @@ int ensure_para(ccfv_ctx *c, int ns)
         (e = hipMalloc(&c->Wk, fb)) != hipSuccess ||
         (e = hipMalloc(&c->partial, sizeof(double) * 2 * c->chain_blocks *
                                         (size_t)(ns + 1))) != hipSuccess ||
-        (e = hipMalloc(&c->slot_sums, sizeof(double) * 2 * (ns + 1))) !=
-            hipSuccess ||
-        (e = hipHostMalloc(&c->h_sums, sizeof(double) * 2 * (ns + 1))) !=
-            hipSuccess) {
+        (e = hipMalloc(&c->slot_sums, sizeof(double) * 2 * (ns + 1) *
+                                          CCFV_MAX_ITERS)) != hipSuccess ||
+        (e = hipHostMalloc(&c->h_sums, sizeof(double) * 2 * (ns + 1) *
+                                           CCFV_MAX_ITERS)) != hipSuccess) {
         free_para(c);
         return fail(CCFV_ENOMEM, std::string("Parareal buffers: ") +
                                      hipGetErrorString(e));
@@ -578,6 +599,8 @@ int ccfv_create(const ccfv_problem *p, ccfv_ctx **out)
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) !=
             hipSuccess ||
+        (e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking)) !=
+            hipSuccess ||
         (e = hipMalloc(&c->state, sizeof(double) * c->ncell)) != hipSuccess ||
         (e = hipMalloc(&c->d_flag, sizeof(int))) != hipSuccess ||
         (e = hipHostMalloc(&c->h_flag, sizeof(int))) != hipSuccess ||
@@ -588,6 +611,11 @@ int ccfv_create(const ccfv_problem *p, ccfv_ctx **out)
                                      hipGetErrorString(e));
     }
     *c->h_flag = 0;
+    c->cur = c->stream;
+    if (const char *s = getenv("CCFV_PIPE")) c->pipeline = atoi(s) != 0;
+    if (const char *s = getenv("CCFV_PIPE_GROUP")) c->pipe_group = std::max(1, atoi(s));
+    if (const char *s = getenv("CCFV_GRAPHS")) c->graphs = atoi(s) != 0;
+    if (const char *s = getenv("CCFV_PLAIN_GRAPHS")) c->plain_graphs = atoi(s) != 0;
     hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream);
     hipMemsetAsync(c->state, 0, sizeof(double) * c->ncell, c->stream);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) {
@@ -601,10 +629,12 @@ int ccfv_create(const ccfv_problem *p, ccfv_ctx **out)
 void ccfv_destroy(ccfv_ctx *c)
 {
     if (!c) return;
-    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     free_para(c);
     drain_records(c);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->sync_ev) (void)hipEventDestroy(e);
     for (auto &kv : c->tables) {
         (void)hipFree(kv.second.dev);
         (void)hipFree(kv.second.line_dev);
@@ -614,7 +644,8 @@ void ccfv_destroy(ccfv_ctx *c)
     if (c->h_flag) hipHostFree(c->h_flag);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -725,8 +756,41 @@ int ccfv_plain_solve(ccfv_ctx *c, double t0, double t1, int32_t nt)
     if (!(t1 > t0) || !std::isfinite(t0) || !std::isfinite(t1))
         return fail(CCFV_EINVAL, "need finite t1 > t0");
     double dt = (t1 - t0) / nt;
-    for (int n = 0; n < nt; n++) {
-        int rc = step_fields(c, c->state, c->state, 1, t0 + n * dt, 0.0, 0, 0.0, dt);
+    c->cur = c->stream;
+    auto body = [&]() -> int {
+        for (int n = 0; n < nt; n++) {
+            int rc = step_fields(c, c->state, c->state, 1, t0 + n * dt, 0.0, 0, 0.0, dt);
+            if (rc) return rc;
+        }
+        return CCFV_OK;
+    };
+    if (c->plain_graphs && !c->prof) {
+        // the whole solve as one hipGraph, replayed by later identical solves
+        const SolveTable *tt;
+        int rc;
+        if ((rc = get_table(c, DIR_X, dt, &tt)) || (rc = get_table(c, DIR_Y, dt, &tt)))
+            return rc;
+        char key[160];
+        snprintf(key, sizeof(key), "pl|%a|%a|%d|%d", t0, t1, nt, c->tiles_version);
+        auto it = c->gcache.find(key);
+        if (it == c->gcache.end()) {
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+            int r2 = body();
+            hipGraph_t gr = nullptr;
+            hipError_t e = hipStreamEndCapture(c->stream, &gr);
+            if (r2 || e != hipSuccess) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return r2 ? r2 : hip_fail(e, "plain-solve graph capture");
+            }
+            hipGraphExec_t x = nullptr;
+            e = hipGraphInstantiate(&x, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+            it = c->gcache.emplace(key, x).first;
+        }
+        HIPCHK(hipGraphLaunch(it->second, c->stream));
+    } else {
+        int rc = body();
         if (rc) return rc;
     }
     return check_flag(c);
@@ -768,6 +832,9 @@ int ccfv_set_tiles(ccfv_ctx *c, const int32_t t[8])
             return fail(CCFV_EINVAL, "invalid tile shape for this grid");
     }
     HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream2));
+    clear_graphs(c);
+    c->tiles_version++;
     c->tile[DIR_X][0] = v[0];
     c->tile[DIR_X][1] = v[1];
     c->tile[DIR_Y][0] = v[2];
@@ -846,6 +913,8 @@ int ccfv_autotune(ccfv_ctx *c, int32_t batch_fields, double h, int32_t out[8])
             c->tile[d][b] = bt;
         }
     }
+    clear_graphs(c);
+    c->tiles_version++;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipStreamSynchronize(c->stream);
@@ -859,6 +928,7 @@ int ccfv_profile_enable(ccfv_ctx *c, int32_t enable)
 {
     if (!c) return fail(CCFV_EINVAL, "NULL context");
     HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream2));
     drain_records(c);
     memset(c->stats, 0, sizeof(c->stats));
     for (int k = 0; k < CCFV_KSTAT_COUNT; k++) c->stats[k].kind = k;
@@ -870,6 +940,7 @@ int ccfv_profile_read(ccfv_ctx *c, ccfv_kernel_stat *out)
 {
     if (!c || !out) return fail(CCFV_EINVAL, "NULL argument");
     HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream2));
     drain_records(c);
     for (int k = 0; k < CCFV_KSTAT_COUNT; k++) {
         out[k] = c->stats[k];
@@ -885,6 +956,302 @@ int ccfv_sync(ccfv_ctx *c)
     return CCFV_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// relative L2 increment of iteration k from the per-slot sums: slots up to k
+// are unchanged (their |U|^2 comes from earlier iterations)
+double increment_of(const ccfv_ctx *c, int ns, int k, std::vector<double> &slot_u2)
+{
+    const double *h = c->h_sums + (size_t)k * 2 * (ns + 1);
+    double num = 0.0, den = 0.0;
+    for (int s = 1; s <= ns; s++) {
+        if (s >= k + 1) {
+            num += h[2 * s];
+            slot_u2[s] = h[2 * s + 1];
+        }
+        den += slot_u2[s];
+    }
+    return den > 0 ? std::sqrt(num / den) : std::sqrt(num);
+}
+
+struct ParaPlan {
+    double t0, DT, dt;
+    int J, ns, kmax;
+    double tol;
+};
+
+// Parareal with everything on one stream: k=0 coarse sweep, then per
+// iteration the batched fine propagators of the slices that can still change
+// and the coarse chain with the correction fused into its last sweep.
+int parareal_sequential(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R,
+                        std::vector<double> &slot_u2, bool &converged, int &kdone)
+{
+    const size_t N = c->ncell;
+    const int ns = pl.ns, J = pl.J, nb = c->chain_blocks;
+    int rc;
+    c->cur = c->stream;
+    for (int n = 0; n < ns; n++) {
+        rc = step_fields(c, c->U + n * N, c->Wk, 1, pl.t0, pl.DT, n, 0.0, pl.DT,
+                         EPI_INIT, c->G + n * N, nullptr, c->U + (n + 1) * N, nullptr);
+        if (rc) return rc;
+    }
+    R.coarse_steps_run = ns;
+    int k;
+    for (k = 0; k < pl.kmax; k++) {
+        const int nf = ns - k;
+        for (int j = 0; j < J; j++) {
+            const double *src = j == 0 ? c->U + k * N : c->F + k * N;
+            rc = step_fields(c, src, c->F + k * N, nf, pl.t0, pl.DT, k, j * pl.dt, pl.dt);
+            if (rc) return rc;
+        }
+        R.fine_steps_run += (double)nf * J;
+        HIPCHK(launch_correction(c->G + k * N, c->G + k * N, c->F + k * N,
+                                 c->U + (k + 1) * N, N,
+                                 c->partial + 2 * (size_t)nb * (k + 1), nb, c->stream));
+        for (int n = k + 1; n < ns; n++) {
+            rc = step_fields(c, c->U + n * N, c->Wk, 1, pl.t0, pl.DT, n, 0.0, pl.DT,
+                             EPI_CORRECT, c->G + n * N, c->F + n * N,
+                             c->U + (n + 1) * N, c->partial + 2 * (size_t)nb * (n + 1));
+            if (rc) return rc;
+        }
+        R.coarse_steps_run += ns - k - 1;
+        double *sums = c->slot_sums + (size_t)k * 2 * (ns + 1);
+        HIPCHK(launch_reduce_segments(c->partial, nb, k + 1, ns - k, sums, c->stream));
+        HIPCHK(hipMemcpyAsync(c->h_sums + (size_t)k * 2 * (ns + 1), sums,
+                              sizeof(double) * 2 * (ns + 1), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        double inc = increment_of(c, ns, k, slot_u2);
+        R.increment[k] = inc;
+        if (!std::isfinite(inc)) {
+            kdone = k + 1;
+            return fail(CCFV_ENONFINITE, "Parareal iterate is not finite");
+        }
+        if (pl.tol > 0 && inc <= pl.tol) {
+            converged = true;
+            k++;
+            break;
+        }
+    }
+    kdone = k;
+    return CCFV_OK;
+}
+
+hipEvent_t sync_event(ccfv_ctx *c, size_t i)
+{
+    while (c->sync_ev.size() <= i) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            return nullptr;
+        c->sync_ev.push_back(e);
+    }
+    return c->sync_ev[i];
+}
+
+// Pipelined Parareal.  Slices form groups of pipe_group.  The coarse chain
+// runs on the context's stream, the fine propagators on a second stream:
+// the fine propagators of iteration k+1 for a group start as soon as the
+// chain of iteration k has written that group's slots (and consumed its F),
+// so the latency-bound chain overlaps the bandwidth-bound fine batches.
+// Fine work of iteration k+1 is issued before iteration k's increment is
+// known; it only writes F, so it is discarded if k converged.  The kernels
+// and their inputs are those of parareal_sequential: results are identical.
+//
+// With graphs on, each phase {chain(k) || fine(k+1)} (and the first phase
+// {k=0 coarse sweep || fine(0)}) is captured once as one hipGraph with a
+// fork/join between the streams and replayed on later solves with the same
+// plan; the dependencies of a phase on the previous one follow from the
+// graph launches' order on the chain stream.
+int parareal_pipelined(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R,
+                       std::vector<double> &slot_u2, bool &converged, int &kdone)
+{
+    const size_t N = c->ncell;
+    const int ns = pl.ns, J = pl.J, nb = c->chain_blocks;
+    const int gs = std::min(c->pipe_group, ns);
+    const int G = (ns + gs - 1) / gs;
+    const bool graphs = c->graphs && !c->prof;
+    auto g0 = [&](int g) { return g * gs; };
+    auto g1 = [&](int g) { return std::min(ns, (g + 1) * gs); };
+    auto grp = [&](int n) { return n / gs; };
+    // events: [par][kind][g], kind 0 = chain passed g, 1 = fine read U of g,
+    // 2 = fine done with g; plus fork / join events
+    auto ev = [&](int par, int kind, int g) { return c->sync_ev[(size_t)((par * 3 + kind) * G + g)]; };
+    for (int i = 0; i < 6 * G + 2; i++)
+        if (!sync_event(c, (size_t)i)) return fail(CCFV_EDEVICE, "event creation failed");
+    hipEvent_t fork = c->sync_ev[(size_t)6 * G], join = c->sync_ev[(size_t)6 * G + 1];
+    hipStream_t sC = c->stream, sF = c->stream2;
+    int rc;
+
+    auto enqueue_fine = [&](int k) -> int {
+        const int par = k & 1;
+        c->cur = sF;
+        for (int g = 0; g < G; g++) {
+            const int lo = std::max(g0(g), k);
+            if (lo >= g1(g)) continue;
+            HIPCHK(hipStreamWaitEvent(sF, ev(par, 0, g), 0));
+            for (int j = 0; j < J; j++) {
+                const double *src = j == 0 ? c->U + lo * N : c->F + lo * N;
+                int r2 = step_fields(c, src, c->F + lo * N, g1(g) - lo, pl.t0, pl.DT,
+                                     lo, j * pl.dt, pl.dt);
+                if (r2) return r2;
+                if (j == 0 && !graphs) HIPCHK(hipEventRecord(ev(par, 1, g), sF));
+            }
+            if (!graphs) HIPCHK(hipEventRecord(ev(par, 2, g), sF));
+        }
+        c->cur = sC;
+        return CCFV_OK;
+    };
+    auto fine_count = [&](int k) {
+        double s = 0;
+        for (int g = 0; g < G; g++) {
+            const int lo = std::max(g0(g), k);
+            if (lo < g1(g)) s += (double)(g1(g) - lo) * J;
+        }
+        return s;
+    };
+    // k = 0 coarse sweep; group g is ready for fine(0) after step g1-1
+    auto enqueue_init = [&]() -> int {
+        c->cur = sC;
+        for (int n = 0; n < ns; n++) {
+            int r2 = step_fields(c, c->U + n * N, c->Wk, 1, pl.t0, pl.DT, n, 0.0, pl.DT,
+                                 EPI_INIT, c->G + n * N, nullptr, c->U + (n + 1) * N,
+                                 nullptr);
+            if (r2) return r2;
+            if (n == g1(grp(n)) - 1) HIPCHK(hipEventRecord(ev(0, 0, grp(n)), sC));
+        }
+        return CCFV_OK;
+    };
+    auto enqueue_chain = [&](int k) -> int {
+        const int par = k & 1, npar = (k + 1) & 1;
+        c->cur = sC;
+        // slot k+1: needs F[k], and fine(k) of the next group must have read
+        // U[k+1] before it is overwritten (implied by the phase order when
+        // graphs are on)
+        if (!graphs) {
+            HIPCHK(hipStreamWaitEvent(sC, ev(par, 2, grp(k)), 0));
+            if (k + 1 < ns) HIPCHK(hipStreamWaitEvent(sC, ev(par, 1, grp(k + 1)), 0));
+        }
+        HIPCHK(launch_correction(c->G + k * N, c->G + k * N, c->F + k * N,
+                                 c->U + (k + 1) * N, N,
+                                 c->partial + 2 * (size_t)nb * (k + 1), nb, sC));
+        for (int n = k + 1; n < ns; n++) {
+            const int g = grp(n);
+            if (!graphs && (n == k + 1 || n == g0(g))) {
+                HIPCHK(hipStreamWaitEvent(sC, ev(par, 2, g), 0));
+                if (g + 1 < G) HIPCHK(hipStreamWaitEvent(sC, ev(par, 1, g + 1), 0));
+            }
+            int r2 = step_fields(c, c->U + n * N, c->Wk, 1, pl.t0, pl.DT, n, 0.0,
+                                 pl.DT, EPI_CORRECT, c->G + n * N, c->F + n * N,
+                                 c->U + (n + 1) * N,
+                                 c->partial + 2 * (size_t)nb * (n + 1));
+            if (r2) return r2;
+            if (n == g1(g) - 1) HIPCHK(hipEventRecord(ev(npar, 0, g), sC));
+        }
+        double *sums = c->slot_sums + (size_t)k * 2 * (ns + 1);
+        HIPCHK(launch_reduce_segments(c->partial, nb, k + 1, ns - k, sums, sC));
+        HIPCHK(hipMemcpyAsync(c->h_sums + (size_t)k * 2 * (ns + 1), sums,
+                              sizeof(double) * 2 * (ns + 1), hipMemcpyDeviceToHost, sC));
+        return CCFV_OK;
+    };
+    // one phase: captured and cached as a graph, or issued directly
+    auto phase = [&](int k, auto &&body) -> int {
+        if (!graphs) return body();
+        char key[256];
+        snprintf(key, sizeof(key), "pa|%a|%a|%a|%d|%d|%d|%d|%d|%d", pl.t0, pl.DT, pl.dt,
+                 J, ns, k, (k + 1 < pl.kmax) ? 1 : 0, gs, c->tiles_version);
+        auto it = c->gcache.find(key);
+        if (it == c->gcache.end()) {
+            HIPCHK(hipStreamBeginCapture(sC, hipStreamCaptureModeRelaxed));
+            hipError_t e = hipEventRecord(fork, sC);
+            if (e == hipSuccess) e = hipStreamWaitEvent(sF, fork, 0);
+            int r2 = e == hipSuccess ? body() : CCFV_EDEVICE;
+            if (r2 == CCFV_OK) {
+                e = hipEventRecord(join, sF);
+                if (e == hipSuccess) e = hipStreamWaitEvent(sC, join, 0);
+            }
+            hipGraph_t gr = nullptr;
+            hipError_t e2 = hipStreamEndCapture(sC, &gr);
+            if (r2 || e != hipSuccess || e2 != hipSuccess) {
+                if (gr) (void)hipGraphDestroy(gr);
+                if (r2) return r2;
+                return hip_fail(e != hipSuccess ? e : e2, "Parareal graph capture");
+            }
+            hipGraphExec_t x = nullptr;
+            e = hipGraphInstantiate(&x, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+            it = c->gcache.emplace(key, x).first;
+        }
+        HIPCHK(hipGraphLaunch(it->second, sC));
+        return CCFV_OK;
+    };
+
+    if (!graphs) {
+        // the fine stream starts after everything queued on the chain stream
+        HIPCHK(hipEventRecord(join, sC));
+        HIPCHK(hipStreamWaitEvent(sF, join, 0));
+    }
+    rc = phase(-1, [&]() -> int {
+        int r2 = enqueue_init();
+        if (r2 == CCFV_OK && pl.kmax > 0) r2 = enqueue_fine(0);
+        return r2;
+    });
+    if (rc) return rc;
+    R.coarse_steps_run = ns;
+    if (pl.kmax > 0) R.fine_steps_run += fine_count(0);
+
+    int k;
+    bool pending = false;  // increments still to be read (tol <= 0)
+    for (k = 0; k < pl.kmax; k++) {
+        rc = phase(k, [&]() -> int {
+            int r2 = enqueue_chain(k);
+            if (r2 == CCFV_OK && k + 1 < pl.kmax) r2 = enqueue_fine(k + 1);
+            return r2;
+        });
+        if (rc) return rc;
+        R.coarse_steps_run += ns - k - 1;
+        if (k + 1 < pl.kmax) R.fine_steps_run += fine_count(k + 1);
+        if (pl.tol > 0) {
+            HIPCHK(hipStreamSynchronize(sC));
+            double inc = increment_of(c, ns, k, slot_u2);
+            R.increment[k] = inc;
+            if (!std::isfinite(inc)) {
+                (void)hipStreamSynchronize(sF);
+                kdone = k + 1;
+                return fail(CCFV_ENONFINITE, "Parareal iterate is not finite");
+            }
+            if (inc <= pl.tol) {
+                converged = true;
+                k++;
+                break;
+            }
+        } else {
+            pending = true;
+        }
+    }
+    kdone = k;
+    if (!graphs) {
+        // join the fine stream (speculative work included) into the chain stream
+        HIPCHK(hipEventRecord(join, sF));
+        HIPCHK(hipStreamWaitEvent(sC, join, 0));
+    }
+    if (pending) {
+        HIPCHK(hipStreamSynchronize(sC));
+        for (int i = 0; i < kdone; i++) {
+            R.increment[i] = increment_of(c, ns, i, slot_u2);
+            if (!std::isfinite(R.increment[i]))
+                return fail(CCFV_ENONFINITE, "Parareal iterate is not finite");
+        }
+    }
+    return CCFV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int ccfv_parareal_solve(ccfv_ctx *c, double t0, double t1, int32_t nt_fine,
                         int32_t ns, int32_t kmax, double tol,
                         ccfv_parareal_report *rep)
@@ -898,98 +1265,88 @@ int ccfv_parareal_solve(ccfv_ctx *c, double t0, double t1, int32_t nt_fine,
     int rc = ensure_para(c, ns);
     if (rc) return rc;
     const size_t N = c->ncell;
-    const double DT = (t1 - t0) / ns;   // :1251
-    const int J = nt_fine / ns;         // :1252
-    const double dt = DT / (double)J;   // :979
-    const int nb = c->chain_blocks;
-    // tables up front (host-side work stays out of the timed loop)
+    ParaPlan pl;
+    pl.t0 = t0;
+    pl.DT = (t1 - t0) / ns;       // :1251
+    pl.J = nt_fine / ns;          // :1252
+    pl.dt = pl.DT / (double)pl.J; // :979
+    pl.ns = ns;
+    pl.kmax = kmax;
+    pl.tol = tol;
+    // tables up front (host-side work stays out of the timed region)
     const SolveTable *tt;
-    if ((rc = get_table(c, DIR_X, DT, &tt)) || (rc = get_table(c, DIR_Y, DT, &tt)) ||
-        (rc = get_table(c, DIR_X, dt, &tt)) || (rc = get_table(c, DIR_Y, dt, &tt)))
+    if ((rc = get_table(c, DIR_X, pl.DT, &tt)) || (rc = get_table(c, DIR_Y, pl.DT, &tt)) ||
+        (rc = get_table(c, DIR_X, pl.dt, &tt)) || (rc = get_table(c, DIR_Y, pl.dt, &tt)))
         return rc;
     ccfv_parareal_report R;
     memset(&R, 0, sizeof(R));
     R.n_slices = ns;
-    R.j_fine = J;
-    std::vector<double> slot_u2(ns + 1, 0.0);  // |U[s]|^2 of the current iterate
+    R.j_fine = pl.J;
+    std::vector<double> slot_u2(ns + 1, 0.0);
+    bool converged = false;
+    int kdone = 0;
 
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     HIPCHK(launch_copy(c->state, c->U, N, c->stream));
-    // k = 0: U[n+1] = G(U[n]), caching G (T2S4_Parareal.cu:1264-1280)
-    for (int n = 0; n < ns; n++) {
-        rc = step_fields(c, c->U + n * N, c->Wk, 1, t0, DT, n, 0.0, DT, EPI_INIT,
-                         c->G + n * N, nullptr, c->U + (n + 1) * N, nullptr);
-        if (rc) return rc;
-    }
-    R.coarse_steps_run = ns;
-    int k = 0;
-    bool converged = false;
-    for (k = 0; k < kmax; k++) {
-        // fine propagators of the slices that can still change, batched:
-        // F[n] = F(U[n]), n = k..ns-1 (T2S4_Parareal.cu:1319-1324)
-        const int nf = ns - k;
-        for (int j = 0; j < J; j++) {
-            const double *src = j == 0 ? c->U + k * N : c->F + k * N;
-            rc = step_fields(c, src, c->F + k * N, nf, t0, DT, k, j * dt, dt);
-            if (rc) return rc;
-        }
-        R.fine_steps_run += (double)nf * J;
-        // slot k+1: U[k] is unchanged, so G(U^{k+1}_k) == G cache (bitwise)
-        HIPCHK(launch_correction(c->G + k * N, c->G + k * N, c->F + k * N,
-                                 c->U + (k + 1) * N, N,
-                                 c->partial + 2 * (size_t)nb * (k + 1), nb,
-                                 c->stream));
-        // coarse chain with the correction fused into the last sweep
-        // (T2S4_Parareal.cu:1302-1330)
-        for (int n = k + 1; n < ns; n++) {
-            rc = step_fields(c, c->U + n * N, c->Wk, 1, t0, DT, n, 0.0, DT,
-                             EPI_CORRECT, c->G + n * N, c->F + n * N,
-                             c->U + (n + 1) * N,
-                             c->partial + 2 * (size_t)nb * (n + 1));
-            if (rc) return rc;
-        }
-        R.coarse_steps_run += ns - k - 1;
-        HIPCHK(launch_reduce_segments(c->partial, nb, k + 1, ns - k, c->slot_sums,
-                                      c->stream));
-        HIPCHK(hipMemcpyAsync(c->h_sums, c->slot_sums,
-                              sizeof(double) * 2 * (ns + 1),
-                              hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        double num = 0.0, den = 0.0;
-        for (int s = 1; s <= ns; s++) {
-            if (s >= k + 1) {
-                num += c->h_sums[2 * s];
-                slot_u2[s] = c->h_sums[2 * s + 1];
-            }
-            den += slot_u2[s];
-        }
-        double inc = den > 0 ? std::sqrt(num / den) : std::sqrt(num);
-        R.increment[k] = inc;
-        if (!std::isfinite(inc)) {
-            k++;
-            R.iterations = k;
-            if (rep) *rep = R;
-            return fail(CCFV_ENONFINITE, "Parareal iterate is not finite");
-        }
-        if (tol > 0 && inc <= tol) {
-            converged = true;
-            k++;
-            break;
-        }
-    }
-    R.iterations = k;
+    rc = c->pipeline ? parareal_pipelined(c, pl, R, slot_u2, converged, kdone)
+                     : parareal_sequential(c, pl, R, slot_u2, converged, kdone);
+    c->cur = c->stream;
+    R.iterations = kdone;
     R.converged = converged ? 1 : 0;
+    if (rc) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamSynchronize(c->stream2);
+        if (rep) *rep = R;
+        return rc;
+    }
     HIPCHK(launch_copy(c->U + (size_t)ns * N, c->state, N, c->stream));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     rc = check_flag(c);
     float ms = 0.f;
-    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
     R.wall_ms = ms;
     if (rep) *rep = R;
     if (rc) return rc;
     if (tol > 0 && !converged)
         return fail(CCFV_ENOTCONV, "Parareal did not reach the tolerance within kmax");
     return CCFV_OK;
+}
+
+int ccfv_set_option(ccfv_ctx *c, int32_t key, int64_t value)
+{
+    if (!c) return fail(CCFV_EINVAL, "NULL context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream2));
+    switch (key) {
+    case CCFV_OPT_PIPELINE:
+        c->pipeline = value != 0;
+        return CCFV_OK;
+    case CCFV_OPT_PIPE_GROUP:
+        if (value < 1 || value > 4096) return fail(CCFV_EINVAL, "group size out of range");
+        c->pipe_group = (int)value;
+        return CCFV_OK;
+    case CCFV_OPT_GRAPHS:
+        c->graphs = value != 0;
+        clear_graphs(c);
+        return CCFV_OK;
+    case CCFV_OPT_PLAIN_GRAPHS:
+        c->plain_graphs = value != 0;
+        clear_graphs(c);
+        return CCFV_OK;
+    }
+    return fail(CCFV_EINVAL, "unknown option");
+}
+
+int ccfv_get_option(ccfv_ctx *c, int32_t key, int64_t *value)
+{
+    if (!c || !value) return fail(CCFV_EINVAL, "NULL argument");
+    switch (key) {
+    case CCFV_OPT_PIPELINE: *value = c->pipeline ? 1 : 0; return CCFV_OK;
+    case CCFV_OPT_PIPE_GROUP: *value = c->pipe_group; return CCFV_OK;
+    case CCFV_OPT_GRAPHS: *value = c->graphs ? 1 : 0; return CCFV_OK;
+    case CCFV_OPT_PLAIN_GRAPHS: *value = c->plain_graphs ? 1 : 0; return CCFV_OK;
+    }
+    return fail(CCFV_EINVAL, "unknown option");
 }
 
 }  // extern "C"

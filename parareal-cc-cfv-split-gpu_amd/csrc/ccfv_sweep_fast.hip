@@ -180,7 +180,7 @@ __global__ __launch_bounds__(256) void fast_x_kernel(SweepArgs a, int W, int NW)
     constexpr bool REG = CH <= 8;
     const int n = a.n, LP = line_pitch(n);
     const size_t foff = (size_t)blockIdx.y * (size_t)a.fstride;
-    const int j0 = blockIdx.x * W;
+    const int j0 = xcd_tile(blockIdx.x, gridDim.x) * W;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     LaneTab<CH, REG> tb;

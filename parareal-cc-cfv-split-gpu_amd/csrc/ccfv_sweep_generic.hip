@@ -65,7 +65,7 @@ __global__ __launch_bounds__(512) void sweep_x_kernel(SweepArgs a, int W, int NW
     extern __shared__ double lds[];
     const int n = a.n, LP = line_pitch(n);
     const size_t foff = (size_t)blockIdx.y * (size_t)a.fstride;
-    const int j0 = blockIdx.x * W;
+    const int j0 = xcd_tile(blockIdx.x, gridDim.x) * W;
     load_tile_x(a.in + foff, a.ld, n, j0, W, lds, LP);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;

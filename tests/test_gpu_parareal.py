@@ -1,0 +1,73 @@
+"""GPU: the pipelined Parareal schedule (fine propagators of iteration k+1
+overlapping the coarse chain of iteration k on a second stream) against the
+single-stream schedule.  Same kernels on the same inputs: bitwise equal."""
+import numpy as np
+import pytest
+
+from ccfv import types as T
+
+pytestmark = pytest.mark.gpu
+
+
+def solve(gpu, p, ic, pipeline, group, kmax, tol, ns=8, nt=64, t1=0.5, graphs=1):
+    with gpu.Context(p) as c:
+        c.fill_exact(ic, 0.0)
+        c.set_option(T.OPT_PIPELINE, pipeline)
+        c.set_option(T.OPT_PIPE_GROUP, group)
+        c.set_option(T.OPT_GRAPHS, graphs)
+        rep = c.parareal_solve(0.0, t1, nt, ns, kmax, tol)
+        return c.get_state(), rep
+
+
+@pytest.mark.parametrize("make", [
+    lambda: (T.rotation_problem(64), T.rotation_ic()),
+    lambda: (T.reference_problem(32, 0.01), T.reference_ic()),
+    lambda: (T.swirl_problem(32, T=1.0), T.cosine_bell_ic()),
+])
+@pytest.mark.parametrize("group", [1, 3, 8])
+@pytest.mark.parametrize("graphs", [0, 1])
+def test_pipelined_equals_sequential(gpu, make, group, graphs):
+    p, ic = make()
+    for kmax, tol in ((4, 0.0), (8, 1e-9)):
+        us, rs = solve(gpu, p, ic, 0, group, kmax, tol)
+        up, rp = solve(gpu, p, ic, 1, group, kmax, tol, graphs=graphs)
+        assert np.array_equal(us, up)
+        assert rs.iterations == rp.iterations
+        assert rs.converged == rp.converged
+        np.testing.assert_allclose(rs.increments, rp.increments, rtol=1e-12)
+
+
+def test_graph_replays_match_eager(gpu, oracle):
+    """Captured phases replayed several times, with eager work and host
+    synchronisation in between, against the eager (uncaptured) solve."""
+    p, ic = T.rotation_problem(64), T.rotation_ic()
+    ref, rref = solve(gpu, p, ic, 1, 3, 4, 0.0, graphs=0)
+    with gpu.Context(p) as c:
+        c.set_option(T.OPT_PIPE_GROUP, 3)
+        c.set_option(T.OPT_GRAPHS, 1)
+        for rep_i in range(4):
+            c.fill_exact(ic, 0.0)
+            c.step(0.0, 1e-3)          # eager work between replays
+            c.fill_exact(ic, 0.0)
+            rep = c.parareal_solve(0.0, 0.5, 64, 8, 4, 0.0)
+            u = c.get_state()
+            assert np.array_equal(u, ref), rep_i
+            np.testing.assert_allclose(rep.increments, rref.increments, rtol=1e-12)
+
+
+def test_early_stop_reports_convergence(gpu):
+    # G == F (one fine step per slice): the first correction changes nothing
+    p, ic = T.rotation_problem(64), T.rotation_ic()
+    u, rep = solve(gpu, p, ic, 1, 2, 8, 1e-12, ns=8, nt=8)
+    assert rep.converged == 1
+    assert rep.iterations == 1
+    assert rep.increments[0] == 0.0
+
+
+def test_not_converged_status(gpu):
+    p, ic = T.rotation_problem(64), T.rotation_ic()
+    with gpu.Context(p) as c:
+        c.fill_exact(ic, 0.0)
+        with pytest.raises(gpu.CcfvError) as ei:
+            c.parareal_solve(0.0, 0.5, 64, 8, 2, 1e-15, raise_on_notconv=True)
+        assert ei.value.code == T.CCFV_ENOTCONV
