@@ -34,6 +34,13 @@ sys.path.insert(0, os.path.join(ROOT, "parareal-cc-cfv-split-gpu_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+# HBM bytes per launch over algorithmic bytes, per kernel class, from two
+# rocprofv3 --pmc passes (FETCH_SIZE x2.0 + WRITE_SIZE x1.0: the scale factors
+# that make copy_kernel of a known size read its true bytes), over one
+# Parareal iteration of this workload (profiles/rotation512_fine_sweep_pmc.md)
+TRAFFIC_RATIO = {"x_batch": 1.003, "y_batch": 1.004, "x_single": 1.06,
+                 "y_single": 1.07, "x_epilog": 1.02}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -49,7 +56,7 @@ def parse():
                     help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe", action="store_true",
-                    help="short run for counter collection: one 128 MiB "
+                    help="short run for counter collection: one 32 MiB "
                          "calibration copy and one Parareal iteration")
     return ap.parse_args()
 
@@ -103,9 +110,9 @@ def main():
 
     p, ic = T.rotation_problem(a.n), T.rotation_ic()
     if a.probe:
-        # calibration: copy_kernel over 4096^2 doubles (128 MiB in, 128 MiB
-        # out) between two fields that are not cache resident
-        big = T.rotation_problem(4096)
+        # calibration: copy_kernel over 2048^2 doubles (32 MiB in, 32 MiB
+        # out) between two fields that are not L2 resident
+        big = T.rotation_problem(2048)
         c1, c2 = ccfv.Context(big), ccfv.Context(big)
         c1.fill_exact(ic, 0.0)
         junk = ccfv.Context(big)
@@ -205,7 +212,9 @@ def main():
                            for k, v in stats.items() if v["launches"]},
         "roofline": {"bound": "hbm", "kernel": dom_name,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (bytes_per_launch * TRAFFIC_RATIO[dom_name]
+                                 if dom_name in TRAFFIC_RATIO else None),
                      "mean_launch_us": mean_ms * 1e3,
                      "bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
