@@ -121,7 +121,8 @@ typedef struct ccfv_ic {
 #define CCFV_MAX_ITERS 64
 typedef struct ccfv_parareal_report {
     int32_t iterations;          /* iterations performed (k)                   */
-    int32_t converged;           /* 1 if the increment fell to <= tol          */
+    int32_t converged;           /* 1 if the increment fell to <= tol, or all  */
+                                 /* n_slices iterations ran (every slice exact) */
     int32_t n_slices, j_fine;    /* slices and fine steps per slice            */
     double  increment[CCFV_MAX_ITERS]; /* rel. L2 increment per iteration over
                                         * all slice end-points                */
@@ -185,6 +186,8 @@ int ccfv_plain_solve(ccfv_ctx *ctx, double t0, double t1, int32_t nt);
  * launch per sweep; the correction is fused into the coarse step.  Stops after
  * kmax iterations or once the relative L2 increment over all slice end-points
  * is <= tol (tol <= 0: run all kmax iterations, as the reference does).
+ * After n_slices iterations every slice end-point is the serial fine solution
+ * and further iterations change nothing, so min(kmax, n_slices) run.
  * The initial state is the context state; on return the state holds U^k(t1).
  * Returns CCFV_OK, or CCFV_ENOTCONV when tol > 0 was not reached (the state
  * still holds the last iterate).  rep may be NULL. */
@@ -259,6 +262,30 @@ typedef struct ccfv_kernel_stat {
 int ccfv_profile_enable(ccfv_ctx *ctx, int32_t enable);
 /* Fill out[CCFV_KSTAT_COUNT]; synchronises the stream. */
 int ccfv_profile_read(ccfv_ctx *ctx, ccfv_kernel_stat *out);
+
+/* ---- batches of independent problems -----------------------------------
+ * (config-4 style workloads; no counterpart in the reference, which solves
+ * one hard-coded problem).  Problems may differ in grid (multiples of 8),
+ * velocity, diffusion, boundary mode, time interval, steps and slices.  Every
+ * sub-step of every problem is issued through ragged launches (one per sweep
+ * stage and line class) in lock-step rounds; each problem's result is
+ * bitwise the one ccfv_parareal_solve / ccfv_plain_solve gives it alone.
+ * Problem i's state is reached through ccfv_batch_context (set/get state,
+ * fill_exact); the batch owns those contexts. */
+typedef struct ccfv_batch ccfv_batch;
+int  ccfv_batch_create(const ccfv_problem *problems, int32_t n, ccfv_batch **out);
+void ccfv_batch_destroy(ccfv_batch *b);
+int  ccfv_batch_size(ccfv_batch *b, int32_t *n);
+int  ccfv_batch_context(ccfv_batch *b, int32_t i, ccfv_ctx **ctx);
+/* plain sweeps: problem i takes nt[i] steps of (t1[i]-t0[i])/nt[i] */
+int  ccfv_batch_plain(ccfv_batch *b, const double *t0, const double *t1,
+                      const int32_t *nt);
+/* Parareal for every problem; each stops on its own increment <= tol (or
+ * after kmax iterations, or when all its slices are exact).  reports[i] (may
+ * be NULL) receives problem i's report; wall_ms is the batch's. */
+int  ccfv_batch_parareal(ccfv_batch *b, const double *t0, const double *t1,
+                         const int32_t *nt_fine, const int32_t *n_slices,
+                         int32_t kmax, double tol, ccfv_parareal_report *reports);
 
 #ifdef __cplusplus
 }

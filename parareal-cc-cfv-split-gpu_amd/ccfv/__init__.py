@@ -76,6 +76,19 @@ _SIGS = {
     "ccfv_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "ccfv_profile_read": (ctypes.c_int, [ctypes.c_void_p,
                                          ctypes.POINTER(types.KernelStat)]),
+    "ccfv_batch_create": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.c_int32,
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "ccfv_batch_destroy": (None, [ctypes.c_void_p]),
+    "ccfv_batch_size": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    "ccfv_batch_context": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
+                                          ctypes.POINTER(ctypes.c_void_p)]),
+    "ccfv_batch_plain": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp,
+                                        ctypes.POINTER(ctypes.c_int32)]),
+    "ccfv_batch_parareal": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp,
+                                           ctypes.POINTER(ctypes.c_int32),
+                                           ctypes.POINTER(ctypes.c_int32),
+                                           ctypes.c_int32, ctypes.c_double,
+                                           ctypes.POINTER(ParaReport)]),
 }
 
 _lib = None
@@ -257,5 +270,98 @@ class Context:
                                             "bytes": s.bytes} for s in arr}
 
 
-__all__ = ["Context", "CcfvError", "load", "device_count", "types", "Problem",
-           "IC", "ParaReport", "exported_symbols"]
+class _MemberContext(Context):
+    """A context owned by a Batch: usable like a Context, never destroyed
+    from Python (ccfv_batch_destroy frees it)."""
+
+    def __init__(self, handle, problem: Problem, owner):
+        self.problem = problem.copy()
+        self._h = handle
+        self._owner = owner  # keeps the batch alive while this view exists
+
+    def close(self):
+        self._h = None
+
+
+def _per_problem(v, n, dtype):
+    a = np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=dtype), (n,)))
+    return a
+
+
+class Batch:
+    """Independent problems of mixed sizes solved together (ccfv_batch_*):
+    every sub-step of every problem goes through shared ragged launches, and
+    each problem's result is bitwise the one its own Context would give."""
+
+    def __init__(self, problems):
+        arr = (Problem * len(problems))(*[p.copy() for p in problems])
+        h = ctypes.c_void_p()
+        _check(load().ccfv_batch_create(arr, len(problems), ctypes.byref(h)),
+               "ccfv_batch_create")
+        self._h = h
+        self.problems = [p.copy() for p in problems]
+        self.contexts = []
+        for i, p in enumerate(self.problems):
+            c = ctypes.c_void_p()
+            _check(load().ccfv_batch_context(self._h, i, ctypes.byref(c)),
+                   "ccfv_batch_context")
+            self.contexts.append(_MemberContext(c, p, self))
+
+    def __len__(self):
+        return len(self.problems)
+
+    def __getitem__(self, i) -> Context:
+        return self.contexts[i]
+
+    def close(self):
+        if self._h:
+            for c in self.contexts:
+                c.close()
+            load().ccfv_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def size(self) -> int:
+        n = ctypes.c_int32()
+        _check(load().ccfv_batch_size(self._h, ctypes.byref(n)), "ccfv_batch_size")
+        return n.value
+
+    def plain(self, t0, t1, nt):
+        n = len(self)
+        a0 = _per_problem(t0, n, np.float64)
+        a1 = _per_problem(t1, n, np.float64)
+        an = _per_problem(nt, n, np.int32)
+        _check(load().ccfv_batch_plain(self._h, _ptr(a0), _ptr(a1),
+                                       an.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+               "ccfv_batch_plain")
+
+    def parareal(self, t0, t1, nt_fine, n_slices, kmax, tol=0.0,
+                 raise_on_notconv=False) -> list:
+        n = len(self)
+        a0 = _per_problem(t0, n, np.float64)
+        a1 = _per_problem(t1, n, np.float64)
+        af = _per_problem(nt_fine, n, np.int32)
+        asl = _per_problem(n_slices, n, np.int32)
+        reps = (ParaReport * n)()
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        rc = load().ccfv_batch_parareal(self._h, _ptr(a0), _ptr(a1),
+                                        af.ctypes.data_as(i32), asl.ctypes.data_as(i32),
+                                        kmax, tol, reps)
+        if not (rc == CCFV_ENOTCONV and not raise_on_notconv):
+            _check(rc, "ccfv_batch_parareal")
+        return list(reps)
+
+
+__all__ = ["Context", "Batch", "CcfvError", "load", "device_count", "types",
+           "Problem", "IC", "ParaReport", "exported_symbols"]

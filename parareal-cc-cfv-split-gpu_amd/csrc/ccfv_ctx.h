@@ -1,0 +1,83 @@
+// ccfv_ctx.h — the solver context and the host runtime helpers shared by
+// the single-problem entry points (ccfv_runtime.hip) and the batch / serving
+// layer (ccfv_batch.hip).
+#pragma once
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "ccfv_internal.h"
+
+struct ccfv_ctx {
+    ccfv_problem P{};
+    int device = 0;
+    hipStream_t stream = nullptr;   // the context's stream (coarse chain)
+    hipStream_t stream2 = nullptr;  // fine propagators when pipelined
+    hipStream_t cur = nullptr;      // stream the next sweep launches go to
+    size_t ncell = 0;
+    double *state = nullptr;
+    int *d_flag = nullptr;   // crossed-feet / non-finite flag
+    int *h_flag = nullptr;   // pinned mirror
+    std::map<std::pair<int, double>, ccfv::SolveTable> tables;
+    // tiles: [dir][0 = single field, 1 = batch]
+    ccfv::Tile tile[2][2];
+    // Parareal buffers
+    int para_ns = 0;
+    double *U = nullptr, *G = nullptr, *F = nullptr, *Wk = nullptr;
+    double *partial = nullptr, *slot_sums = nullptr, *h_sums = nullptr;
+    int chain_blocks = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // pipelined Parareal: fine propagators of iteration k+1 overlap the
+    // coarse chain of iteration k, group by group
+    bool pipeline = true;
+    int pipe_group = 32;
+    // captured Parareal phases, keyed by plan; cleared when buffers or
+    // launch shapes change
+    bool graphs = false;
+    bool plain_graphs = false;
+    int tiles_version = 0;
+    std::map<std::string, hipGraphExec_t> gcache;
+    std::vector<hipEvent_t> sync_ev;  // cross-stream events (no timing)
+    // kernel timing
+    bool prof = false;
+    struct Rec {
+        int kind;
+        double bytes;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> ev_pool;
+    ccfv_kernel_stat stats[CCFV_KSTAT_COUNT];
+};
+
+namespace ccfv {
+namespace rt {
+
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define HIPCHK(expr)                                      \
+    do {                                                  \
+        hipError_t e_ = (expr);                           \
+        if (e_ != hipSuccess) return ccfv::rt::hip_fail(e_, #expr); \
+    } while (0)
+
+bool tile_ok(Tile t, int n, int nl, Dir dir, bool post);
+double cell_width(const ccfv_problem &P, Dir dir);
+int get_table(ccfv_ctx *c, Dir dir, double h, const SolveTable **out);
+SweepArgs make_args(ccfv_ctx *c, Dir dir, int substep, const SolveTable &T,
+                    double h, double t0, double tDT, int field0, double jdt);
+bool post_flag(const ccfv_ctx *c);
+int step_fields(ccfv_ctx *c, const double *in, double *out, int nf, double t0,
+                double tDT, int field0, double jdt, double h, Epi epi = EPI_NONE,
+                double *g_cache = nullptr, const double *f_in = nullptr,
+                double *u_next = nullptr, double *partial = nullptr);
+int check_flag(ccfv_ctx *c);
+int validate_problem(const ccfv_problem *p);
+int ensure_para(ccfv_ctx *c, int ns);
+void clear_graphs(ccfv_ctx *c);
+double increment_of(const ccfv_ctx *c, int ns, int k, std::vector<double> &slot_u2);
+
+}  // namespace rt
+}  // namespace ccfv

@@ -77,6 +77,14 @@ struct Tile {
 int chunk_for(int n);
 bool supported_n(int n);
 
+// One problem's share of a ragged launch: blocks prefix[i]..prefix[i+1]-1 of
+// the launch sweep nfields fields of ntiles line tiles each with args a.
+struct RaggedItem {
+    SweepArgs a;
+    int nfields;
+    int ntiles;
+};
+
 // general sweeps (ccfv_sweep_generic.hip)
 hipError_t launch_sweep_generic(Dir dir, int bc, bool post, Epi epi,
                                 const SweepArgs &a, int nfields, Tile tile,
@@ -86,6 +94,33 @@ size_t sweep_lds_bytes_generic(int n, Tile tile);
 hipError_t launch_sweep_fast(Dir dir, Epi epi, const SweepArgs &a, int nfields,
                              Tile tile, hipStream_t s);
 size_t sweep_lds_bytes_fast(int n, Tile tile);
+// ragged launches over many problems of one line class (same chunk CH,
+// boundary mode and path); items and prefix are device arrays
+hipError_t launch_ragged_fast(Dir dir, Epi epi, int ch, const RaggedItem *items,
+                              const int *prefix, int nitems, int nblocks,
+                              Tile tile, size_t lds, hipStream_t s);
+hipError_t launch_ragged_generic(Dir dir, int bc, bool post, Epi epi, int ch,
+                                 const RaggedItem *items, const int *prefix,
+                                 int nitems, int nblocks, Tile tile, size_t lds,
+                                 hipStream_t s);
+// elementwise corrections of many problems: item i covers blocks
+// prefix[i]..prefix[i+1]-1
+struct RaggedCorr {
+    const double *g_new, *g_old, *f;
+    double *u;
+    size_t n;
+    double *partial;
+};
+hipError_t launch_ragged_correction(const RaggedCorr *items, const int *prefix,
+                                    int nitems, int nblocks, hipStream_t s);
+// per-slot sums of many problems: item i reduces slots first..first+count-1
+struct RaggedReduce {
+    const double *partial;
+    double *out;
+    int seg, first, count;
+};
+hipError_t launch_ragged_reduce(const RaggedReduce *items, const int *prefix,
+                                int nitems, int nblocks, hipStream_t s);
 // element-wise kernels (ccfv_misc.hip)
 hipError_t launch_correction(const double *g_new, const double *g_old,
                              const double *f, double *u, size_t n,

@@ -31,13 +31,13 @@ struct Times {
     double hi, lo;
 };
 
-// sampling times of the velocity for the sub-step of field blockIdx.y:
+// sampling times of the velocity for the sub-step of field f of a launch:
 // x sub-step 1 over [tn, tn+h/2], y over [tn, tn+h], x sub-step 3 over
 // [tn+h/2, tn+h]; tn = t0 + n*DT (+ j*dt inside the fine propagator, :981)
-__device__ __forceinline__ Times sub_times(const SweepArgs &a)
+__device__ __forceinline__ Times sub_times(const SweepArgs &a, int f)
 {
 #pragma clang fp contract(off)
-    double tn = a.t0 + (double)(a.field0 + (int)blockIdx.y) * a.tDT;
+    double tn = a.t0 + (double)(a.field0 + f) * a.tDT;
     double tl = tn + a.jdt;
     Times t;
     if (a.substep == 1) {
@@ -55,10 +55,9 @@ __device__ __forceinline__ Times sub_times(const SweepArgs &a)
 
 // foot of face q of the line with corner coordinates o0, o1
 template <int DIR>
-__device__ __forceinline__ double face_foot(const SweepArgs &a, int q,
-                                            double o0, double o1)
+__device__ __forceinline__ double face_foot(const SweepArgs &a, const Times &t,
+                                            int q, double o0, double o1)
 {
-    Times t = sub_times(a);
     VelParams v{a.vel_kind, a.v0, a.v1, a.v2};
     return face_foot<DIR>(v, a.lo, a.d, a.tau, t.hi, t.lo, q, o0, o1);
 }
@@ -179,16 +178,17 @@ __device__ __forceinline__ double deriv_ref(const SweepArgs &a,
 }
 
 template <int DIR, int BC, bool POST>
-__device__ __forceinline__ Face eval_face(const SweepArgs &a, const double *cs,
-                                          const double g[4], int q, double o0,
-                                          double o1, double inv12d)
+__device__ __forceinline__ Face eval_face(const SweepArgs &a, const Times &tt,
+                                          const double *cs, const double g[4],
+                                          int q, double o0, double o1,
+                                          double inv12d)
 {
     Face f;
     const int n = a.n;
     if (BC == BC_PER) {
         // face n is face 0 one period on (exact periodicity of the fluxes)
         int qq = q == n ? 0 : q;
-        f.xb = face_foot<DIR>(a, qq, o0, o1);
+        f.xb = face_foot<DIR>(a, tt, qq, o0, o1);
         double xi;
         int m = cell_of(a, f.xb, xi);
         int mw = wrapn(m, n);
@@ -202,7 +202,7 @@ __device__ __forceinline__ Face eval_face(const SweepArgs &a, const double *cs,
         f.mL = f.mR = q == n ? m + n : m;
         f.D = POST ? 0.0 : deriv(w, xi, inv12d);
     } else {
-        f.xb = face_foot<DIR>(a, q, o0, o1);
+        f.xb = face_foot<DIR>(a, tt, q, o0, o1);
         double xl, xr;
         {
 #pragma clang fp contract(off)
@@ -279,19 +279,19 @@ __device__ __forceinline__ double remap_cell(const SweepArgs &a,
 // RHS b of the cells of this lane by the general per-face path; b goes to the
 // lane's own slots of `aux` (an LDS line used as an indexable register file).
 template <int DIR, int BC>
-__device__ void rhs_general(const SweepArgs &a, const double *cs,
+__device__ void rhs_general(const SweepArgs &a, const Times &tt, const double *cs,
                             const double g[4], double *aux, int base, int ch,
                             double o0, double o1, double inv12d, bool post)
 {
     bool bad = false;
-    Face fl = post ? eval_face<DIR, BC, true>(a, cs, g, base, o0, o1, inv12d)
-                   : eval_face<DIR, BC, false>(a, cs, g, base, o0, o1, inv12d);
+    Face fl = post ? eval_face<DIR, BC, true>(a, tt, cs, g, base, o0, o1, inv12d)
+                   : eval_face<DIR, BC, false>(a, tt, cs, g, base, o0, o1, inv12d);
 #pragma unroll 1
     for (int k = 0; k < ch; k++) {
-        Face fr = post ? eval_face<DIR, BC, true>(a, cs, g, base + k + 1, o0, o1,
-                                                  inv12d)
-                       : eval_face<DIR, BC, false>(a, cs, g, base + k + 1, o0, o1,
-                                                   inv12d);
+        Face fr = post ? eval_face<DIR, BC, true>(a, tt, cs, g, base + k + 1, o0,
+                                                  o1, inv12d)
+                       : eval_face<DIR, BC, false>(a, tt, cs, g, base + k + 1, o0,
+                                                   o1, inv12d);
         double cbar = remap_cell<BC>(a, cs, fl, fr, bad);
         aux[pidx(base + k)] = post ? cbar : cbar + a.coef * (fr.D - fl.D);
         fl = fr;
@@ -300,7 +300,7 @@ __device__ void rhs_general(const SweepArgs &a, const double *cs,
 }
 
 // ---------------------------------------------------------------------------
-// block reduction of two doubles into partial[2*blockIdx.x ...]
+// block reduction of two doubles into partial[2*slot ...]
 // ---------------------------------------------------------------------------
 __device__ inline void block_reduce2(double s0, double s1, double *partial, int slot)
 {
@@ -327,6 +327,18 @@ __device__ inline void block_reduce2(double s0, double s1, double *partial, int 
     }
 }
 
+
+// item of block b in a ragged launch: prefix[i] <= b < prefix[i+1]
+__device__ __forceinline__ int ragged_item(const int *prefix, int nitems, int b)
+{
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
 
 // ---------------------------------------------------------------------------
 // compact operator and implicit half, shared by both paths
@@ -482,7 +494,8 @@ __device__ __forceinline__ void store_tile_x(double *out, int ld, int n, int j0,
 //                G cache <- g, block partials of |dU|^2 and |U|^2
 template <int EPI>
 __device__ __forceinline__ void epilogue_x(const SweepArgs &a, int n, int j0,
-                                           int W, const double *lds, int LP)
+                                           int W, const double *lds, int LP,
+                                           int slot)
 {
     const int hw = W >> 1, sh = log2i(hw);
     double s0 = 0.0, s1 = 0.0;
@@ -509,7 +522,7 @@ __device__ __forceinline__ void epilogue_x(const SweepArgs &a, int n, int j0,
             s1 += un.x * un.x + un.y * un.y;
         }
     }
-    if (EPI == EPI_CORRECT) block_reduce2(s0, s1, a.partial, blockIdx.x);
+    if (EPI == EPI_CORRECT) block_reduce2(s0, s1, a.partial, slot);
 }
 
 // y lines are rows i0..i0+W-1 (contiguous)

@@ -82,29 +82,99 @@ hipError_t launch_reduce_partials(const double *partial, int count, double *out2
     return hipGetLastError();
 }
 
-// one thread per slot sums its segment in a fixed order
-__global__ void reduce_segments_kernel(const double *partial, int seg, int first,
-                                       int count, double *out)
+// one wavefront per slot: lane l sums partials l, l+64, ... in order, then a
+// fixed butterfly across lanes (deterministic for a given segment length)
+__global__ __launch_bounds__(64) void reduce_segments_kernel(
+    const double *partial, int seg, int first, double *out)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    int s = first + t;
+    const int s = first + blockIdx.x, lane = threadIdx.x;
     const double *p = partial + 2 * (size_t)seg * s;
     double s0 = 0.0, s1 = 0.0;
-    for (int i = 0; i < seg; i++) {
+    for (int i = lane; i < seg; i += 64) {
         s0 += p[2 * i];
         s1 += p[2 * i + 1];
     }
-    out[2 * s] = s0;
-    out[2 * s + 1] = s1;
+    for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off);
+        s1 += __shfl_xor(s1, off);
+    }
+    if (lane == 0) {
+        out[2 * s] = s0;
+        out[2 * s + 1] = s1;
+    }
 }
 
 hipError_t launch_reduce_segments(const double *partial, int seg, int first,
                                   int count, double *out, hipStream_t s)
 {
     if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(reduce_segments_kernel, dim3((count + 63) / 64), dim3(64),
-                       0, s, partial, seg, first, count, out);
+    hipLaunchKernelGGL(reduce_segments_kernel, dim3(count), dim3(64), 0, s,
+                       partial, seg, first, out);
+    return hipGetLastError();
+}
+
+// ragged elementwise correction: block b works for item i on element blocks
+// b - prefix[i] of prefix[i+1] - prefix[i]
+__global__ __launch_bounds__(256) void ragged_correction_kernel(
+    const RaggedCorr *items, const int *prefix, int nitems)
+{
+    const int b = blockIdx.x;
+    const int i = __builtin_amdgcn_readfirstlane(ragged_item(prefix, nitems, b));
+    const RaggedCorr it = items[i];
+    const int nb = prefix[i + 1] - prefix[i], lb = b - prefix[i];
+    double s0 = 0.0, s1 = 0.0;
+    for (size_t e = lb * (size_t)blockDim.x + threadIdx.x; e < it.n;
+         e += (size_t)nb * blockDim.x) {
+        double un = it.g_new[e] + (it.f[e] - it.g_old[e]);
+        if (it.partial) {
+            double d = un - it.u[e];
+            s0 += d * d;
+            s1 += un * un;
+        }
+        it.u[e] = un;
+    }
+    if (it.partial) block_reduce2(s0, s1, it.partial, lb);
+}
+
+hipError_t launch_ragged_correction(const RaggedCorr *items, const int *prefix,
+                                    int nitems, int nblocks, hipStream_t s)
+{
+    if (nblocks <= 0 || nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ragged_correction_kernel, dim3(nblocks), dim3(256), 0, s, items,
+                       prefix, nitems);
+    return hipGetLastError();
+}
+
+// ragged slot sums: block b reduces slot (b - prefix[i]) of item i
+__global__ __launch_bounds__(64) void ragged_reduce_kernel(const RaggedReduce *items,
+                                                           const int *prefix, int nitems)
+{
+    const int b = blockIdx.x;
+    const int i = __builtin_amdgcn_readfirstlane(ragged_item(prefix, nitems, b));
+    const RaggedReduce it = items[i];
+    const int s = it.first + (b - prefix[i]), lane = threadIdx.x;
+    const double *p = it.partial + 2 * (size_t)it.seg * s;
+    double s0 = 0.0, s1 = 0.0;
+    for (int k = lane; k < it.seg; k += 64) {
+        s0 += p[2 * k];
+        s1 += p[2 * k + 1];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off);
+        s1 += __shfl_xor(s1, off);
+    }
+    if (lane == 0) {
+        it.out[2 * s] = s0;
+        it.out[2 * s + 1] = s1;
+    }
+}
+
+hipError_t launch_ragged_reduce(const RaggedReduce *items, const int *prefix,
+                                int nitems, int nblocks, hipStream_t s)
+{
+    if (nblocks <= 0 || nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ragged_reduce_kernel, dim3(nblocks), dim3(64), 0, s, items,
+                       prefix, nitems);
     return hipGetLastError();
 }
 

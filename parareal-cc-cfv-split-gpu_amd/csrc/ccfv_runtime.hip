@@ -4,8 +4,8 @@
 // once Parareal runs, the slice arrays.  All work is queued on the context's
 // own stream (default priority).  Host code never touches field data except
 // through ccfv_set_state / ccfv_get_state.
+#include "ccfv_ctx.h"
 #include "ccfv_feet.h"
-#include "ccfv_internal.h"
 
 #include <algorithm>
 #include <cmath>
@@ -18,8 +18,10 @@
 #include <vector>
 
 using namespace ccfv;
+using namespace ccfv::rt;
 
-namespace {
+namespace ccfv {
+namespace rt {
 
 thread_local std::string g_err;
 
@@ -34,57 +36,12 @@ int hip_fail(hipError_t e, const char *what)
     return fail(CCFV_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define HIPCHK(expr)                                  \
-    do {                                              \
-        hipError_t e_ = (expr);                       \
-        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
-    } while (0)
+}  // namespace rt
+}  // namespace ccfv
 
-}  // namespace
 
-struct ccfv_ctx {
-    ccfv_problem P{};
-    int device = 0;
-    hipStream_t stream = nullptr;   // the context's stream (coarse chain)
-    hipStream_t stream2 = nullptr;  // fine propagators when pipelined
-    hipStream_t cur = nullptr;      // stream the next sweep launches go to
-    size_t ncell = 0;
-    double *state = nullptr;
-    int *d_flag = nullptr;   // crossed-feet / non-finite flag
-    int *h_flag = nullptr;   // pinned mirror
-    std::map<std::pair<int, double>, SolveTable> tables;
-    // tiles: [dir][0 = single field, 1 = batch]
-    Tile tile[2][2];
-    // Parareal buffers
-    int para_ns = 0;
-    double *U = nullptr, *G = nullptr, *F = nullptr, *Wk = nullptr;
-    double *partial = nullptr, *slot_sums = nullptr, *h_sums = nullptr;
-    int chain_blocks = 0;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // pipelined Parareal: fine propagators of iteration k+1 overlap the
-    // coarse chain of iteration k, group by group
-    bool pipeline = true;
-    int pipe_group = 32;
-    // captured Parareal phases, keyed by plan; cleared when buffers or
-    // launch shapes change
-    bool graphs = false;
-    bool plain_graphs = false;
-    int tiles_version = 0;
-    std::map<std::string, hipGraphExec_t> gcache;
-    std::vector<hipEvent_t> sync_ev;  // cross-stream events (no timing)
-    // kernel timing
-    bool prof = false;
-    struct Rec {
-        int kind;
-        double bytes;
-        hipEvent_t a, b;
-    };
-    std::vector<Rec> recs;
-    std::vector<hipEvent_t> ev_pool;
-    ccfv_kernel_stat stats[CCFV_KSTAT_COUNT];
-};
-
-namespace {
+namespace ccfv {
+namespace rt {
 
 Tile env_tile(const char *name, Tile dflt)
 {
@@ -382,9 +339,8 @@ bool post_flag(const ccfv_ctx *c)
 // T2S4_Parareal.cu:785-968).  The first sweep reads `in` and writes `out`;
 // the other two work in place on `out` (every sweep is line-local).
 int step_fields(ccfv_ctx *c, const double *in, double *out, int nf, double t0,
-                double tDT, int field0, double jdt, double h, Epi epi = EPI_NONE,
-                double *g_cache = nullptr, const double *f_in = nullptr,
-                double *u_next = nullptr, double *partial = nullptr)
+                double tDT, int field0, double jdt, double h, Epi epi,
+                double *g_cache, const double *f_in, double *u_next, double *partial)
 {
     const SolveTable *tx, *ty;
     int rc = get_table(c, DIR_X, h, &tx);
@@ -512,7 +468,8 @@ int ensure_para(ccfv_ctx *c, int ns)
     return CCFV_OK;
 }
 
-}  // namespace
+}  // namespace rt
+}  // namespace ccfv
 
 extern "C" {
 
@@ -958,7 +915,8 @@ int ccfv_sync(ccfv_ctx *c)
 
 }  // extern "C"
 
-namespace {
+namespace ccfv {
+namespace rt {
 
 // relative L2 increment of iteration k from the per-slot sums: slots up to k
 // are unchanged (their |U|^2 comes from earlier iterations)
@@ -1248,7 +1206,8 @@ int parareal_pipelined(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R,
     return CCFV_OK;
 }
 
-}  // namespace
+}  // namespace rt
+}  // namespace ccfv
 
 extern "C" {
 
@@ -1271,7 +1230,8 @@ int ccfv_parareal_solve(ccfv_ctx *c, double t0, double t1, int32_t nt_fine,
     pl.J = nt_fine / ns;          // :1252
     pl.dt = pl.DT / (double)pl.J; // :979
     pl.ns = ns;
-    pl.kmax = kmax;
+    // iteration k fixes slot k+1; past n_slices iterations nothing changes
+    pl.kmax = std::min((int)kmax, (int)ns);
     pl.tol = tol;
     // tables up front (host-side work stays out of the timed region)
     const SolveTable *tt;
@@ -1291,6 +1251,7 @@ int ccfv_parareal_solve(ccfv_ctx *c, double t0, double t1, int32_t nt_fine,
     rc = c->pipeline ? parareal_pipelined(c, pl, R, slot_u2, converged, kdone)
                      : parareal_sequential(c, pl, R, slot_u2, converged, kdone);
     c->cur = c->stream;
+    if (rc == CCFV_OK && kdone == ns) converged = true;  // every slice exact
     R.iterations = kdone;
     R.converged = converged ? 1 : 0;
     if (rc) {

@@ -174,13 +174,13 @@ __device__ __forceinline__ void fast_line(const SweepArgs &a, double *cs,
 }
 
 template <int CH, int EPI>
-__global__ __launch_bounds__(256) void fast_x_kernel(SweepArgs a, int W, int NW)
+__device__ __forceinline__ void fast_x_body(const SweepArgs &a, int field, int tile,
+                                           int W, int NW, double *lds)
 {
-    extern __shared__ double lds[];
     constexpr bool REG = CH <= 8;
     const int n = a.n, LP = line_pitch(n);
-    const size_t foff = (size_t)blockIdx.y * (size_t)a.fstride;
-    const int j0 = xcd_tile(blockIdx.x, gridDim.x) * W;
+    const size_t foff = (size_t)field * (size_t)a.fstride;
+    const int j0 = tile * W;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     LaneTab<CH, REG> tb;
@@ -193,17 +193,17 @@ __global__ __launch_bounds__(256) void fast_x_kernel(SweepArgs a, int W, int NW)
         fast_line<CH, REG>(a, lds + l * LP, j0 + l, lane, tb, mfl, mbl, pneg0);
     __syncthreads();
     if (EPI == EPI_NONE) store_tile_x(a.out + foff, a.ld, n, j0, W, lds, LP);
-    else epilogue_x<EPI>(a, n, j0, W, lds, LP);
+    else epilogue_x<EPI>(a, n, j0, W, lds, LP, tile);
 }
 
 template <int CH>
-__global__ __launch_bounds__(256) void fast_y_kernel(SweepArgs a, int W, int NW)
+__device__ __forceinline__ void fast_y_body(const SweepArgs &a, int field, int tile,
+                                           int W, int NW, double *lds)
 {
-    extern __shared__ double lds[];
     constexpr bool REG = CH <= 8;
     const int n = a.n, LP = line_pitch(n);
-    const size_t foff = (size_t)blockIdx.y * (size_t)a.fstride;
-    const int i0 = blockIdx.x * W;
+    const size_t foff = (size_t)field * (size_t)a.fstride;
+    const int i0 = tile * W;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     LaneTab<CH, REG> tb;
@@ -216,6 +216,51 @@ __global__ __launch_bounds__(256) void fast_y_kernel(SweepArgs a, int W, int NW)
         fast_line<CH, REG>(a, lds + l * LP, i0 + l, lane, tb, mfl, mbl, pneg0);
     __syncthreads();
     store_tile_y(a.out + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
+}
+
+template <int CH, int EPI>
+__global__ __launch_bounds__(256) void fast_x_kernel(SweepArgs a, int W, int NW)
+{
+    extern __shared__ double lds[];
+    fast_x_body<CH, EPI>(a, blockIdx.y, xcd_tile(blockIdx.x, gridDim.x), W, NW, lds);
+}
+
+template <int CH>
+__global__ __launch_bounds__(256) void fast_y_kernel(SweepArgs a, int W, int NW)
+{
+    extern __shared__ double lds[];
+    fast_y_body<CH>(a, blockIdx.y, blockIdx.x, W, NW, lds);
+}
+
+// ragged: one launch over many problems of the same line class
+template <int CH, int EPI>
+__global__ __launch_bounds__(256) void ragged_fast_x(const RaggedItem *items,
+                                                      const int *prefix, int nitems,
+                                                      int W, int NW)
+{
+    extern __shared__ double lds[];
+    const int b = blockIdx.x;
+    const int i = __builtin_amdgcn_readfirstlane(ragged_item(prefix, nitems, b));
+    const int local = b - prefix[i];
+    const int nt = items[i].ntiles;
+    const int field = local / nt;
+    const SweepArgs a = items[i].a;
+    fast_x_body<CH, EPI>(a, field, local - field * nt, W, NW, lds);
+}
+
+template <int CH>
+__global__ __launch_bounds__(256) void ragged_fast_y(const RaggedItem *items,
+                                                      const int *prefix, int nitems,
+                                                      int W, int NW)
+{
+    extern __shared__ double lds[];
+    const int b = blockIdx.x;
+    const int i = __builtin_amdgcn_readfirstlane(ragged_item(prefix, nitems, b));
+    const int local = b - prefix[i];
+    const int nt = items[i].ntiles;
+    const int field = local / nt;
+    const SweepArgs a = items[i].a;
+    fast_y_body<CH>(a, field, local - field * nt, W, NW, lds);
 }
 
 size_t sweep_lds_bytes_fast(int n, Tile tile)
@@ -260,6 +305,55 @@ hipError_t launch_sweep_fast(Dir dir, Epi epi, const SweepArgs &a, int nfields,
     case 8: return dispatch<8>(dir, epi, a, nfields, tile, s);
     case 16: return dispatch<16>(dir, epi, a, nfields, tile, s);
     case 32: return dispatch<32>(dir, epi, a, nfields, tile, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename K>
+static hipError_t launch_r(K k, int nblocks, Tile t, size_t lds, hipStream_t s,
+                           const RaggedItem *items, const int *prefix, int nitems)
+{
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute(
+            (const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL(k, dim3(nblocks), dim3(64 * t.nw), lds, s, items, prefix,
+                       nitems, t.w, t.nw);
+    return hipGetLastError();
+}
+
+template <int CH>
+static hipError_t dispatch_r(Dir dir, Epi epi, const RaggedItem *items,
+                             const int *prefix, int nitems, int nblocks, Tile t,
+                             size_t lds, hipStream_t s)
+{
+    if (dir == DIR_Y)
+        return launch_r(ragged_fast_y<CH>, nblocks, t, lds, s, items, prefix, nitems);
+    if (epi == EPI_NONE)
+        return launch_r(ragged_fast_x<CH, EPI_NONE>, nblocks, t, lds, s, items, prefix,
+                        nitems);
+    if (epi == EPI_INIT)
+        return launch_r(ragged_fast_x<CH, EPI_INIT>, nblocks, t, lds, s, items, prefix,
+                        nitems);
+    return launch_r(ragged_fast_x<CH, EPI_CORRECT>, nblocks, t, lds, s, items, prefix,
+                    nitems);
+}
+
+hipError_t launch_ragged_fast(Dir dir, Epi epi, int ch, const RaggedItem *items,
+                              const int *prefix, int nitems, int nblocks, Tile t,
+                              size_t lds, hipStream_t s)
+{
+    if (nblocks <= 0 || nitems <= 0) return hipSuccess;
+    switch (ch) {
+    case 1: return dispatch_r<1>(dir, epi, items, prefix, nitems, nblocks, t, lds, s);
+    case 2: return dispatch_r<2>(dir, epi, items, prefix, nitems, nblocks, t, lds, s);
+    case 4: return dispatch_r<4>(dir, epi, items, prefix, nitems, nblocks, t, lds, s);
+    case 8: return dispatch_r<8>(dir, epi, items, prefix, nitems, nblocks, t, lds, s);
+    case 16: return dispatch_r<16>(dir, epi, items, prefix, nitems, nblocks, t, lds, s);
+    case 32: return dispatch_r<32>(dir, epi, items, prefix, nitems, nblocks, t, lds, s);
     }
     return hipErrorInvalidValue;
 }

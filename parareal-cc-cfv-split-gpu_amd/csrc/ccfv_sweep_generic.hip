@@ -14,8 +14,9 @@ namespace ccfv {
 // aux: a line-sized LDS buffer owned by this wave.
 template <int DIR, int BC, int CH, bool POST>
 __device__ void process_line(const SweepArgs &a, double *cs, double *aux,
-                             int line, int lane)
+                             int line, int lane, int field)
 {
+    const Times tt = sub_times(a, field);
     const int n = a.n;
     const int NL = n / CH;
     const bool active = lane < NL;
@@ -26,7 +27,7 @@ __device__ void process_line(const SweepArgs &a, double *cs, double *aux,
     if (BC == BC_REF) ghosts(cs, n, g);
 
     double b[CH];
-    if (active) rhs_general<DIR, BC>(a, cs, g, aux, base, CH, o0, o1, inv12d, POST);
+    if (active) rhs_general<DIR, BC>(a, tt, cs, g, aux, base, CH, o0, o1, inv12d, POST);
     if (POST) {
         // the reference's third sub-step takes the derivative of the remapped
         // line (T2S4_Parareal.cu:929-932)
@@ -34,12 +35,12 @@ __device__ void process_line(const SweepArgs &a, double *cs, double *aux,
         double gs[4] = {0.0, 0.0, 0.0, 0.0};
         if (BC == BC_REF) ghosts(aux, n, gs);
         if (active) {
-            double dl = deriv_ref<BC>(a, aux, gs, face_foot<DIR>(a, base, o0, o1),
+            double dl = deriv_ref<BC>(a, aux, gs, face_foot<DIR>(a, tt, base, o0, o1),
                                       inv12d);
 #pragma unroll
             for (int k = 0; k < CH; k++) {
                 double dr = deriv_ref<BC>(a, aux, gs,
-                                          face_foot<DIR>(a, base + k + 1, o0, o1),
+                                          face_foot<DIR>(a, tt, base + k + 1, o0, o1),
                                           inv12d);
                 b[k] = aux[pidx(base + k)] + a.coef * (dr - dl);
                 dl = dr;
@@ -59,39 +60,87 @@ __device__ void process_line(const SweepArgs &a, double *cs, double *aux,
                               t + 4 * n, t + 4 * n + 64, cs);
 }
 
+// one block: field `field` of the launch, column tile `tile` (W lines)
 template <int BC, int CH, bool POST, int EPI>
-__global__ __launch_bounds__(512) void sweep_x_kernel(SweepArgs a, int W, int NW)
+__device__ __forceinline__ void sweep_x_body(const SweepArgs &a, int field, int tile,
+                                             int W, int NW, double *lds)
 {
-    extern __shared__ double lds[];
     const int n = a.n, LP = line_pitch(n);
-    const size_t foff = (size_t)blockIdx.y * (size_t)a.fstride;
-    const int j0 = xcd_tile(blockIdx.x, gridDim.x) * W;
+    const size_t foff = (size_t)field * (size_t)a.fstride;
+    const int j0 = tile * W;
     load_tile_x(a.in + foff, a.ld, n, j0, W, lds, LP);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double *aux = lds + (W + wave) * LP;
     for (int l = wave; l < W; l += NW)
-        process_line<DIR_X, BC, CH, POST>(a, lds + l * LP, aux, j0 + l, lane);
+        process_line<DIR_X, BC, CH, POST>(a, lds + l * LP, aux, j0 + l, lane, field);
     __syncthreads();
     if (EPI == EPI_NONE) store_tile_x(a.out + foff, a.ld, n, j0, W, lds, LP);
-    else epilogue_x<EPI>(a, n, j0, W, lds, LP);
+    else epilogue_x<EPI>(a, n, j0, W, lds, LP, tile);
+}
+
+template <int BC, int CH>
+__device__ __forceinline__ void sweep_y_body(const SweepArgs &a, int field, int tile,
+                                             int W, int NW, double *lds)
+{
+    const int n = a.n, LP = line_pitch(n);
+    const size_t foff = (size_t)field * (size_t)a.fstride;
+    const int i0 = tile * W;
+    load_tile_y(a.in + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double *aux = lds + (W + wave) * LP;
+    for (int l = wave; l < W; l += NW)
+        process_line<DIR_Y, BC, CH, false>(a, lds + l * LP, aux, i0 + l, lane, field);
+    __syncthreads();
+    store_tile_y(a.out + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
+}
+
+template <int BC, int CH, bool POST, int EPI>
+__global__ __launch_bounds__(512) void sweep_x_kernel(SweepArgs a, int W, int NW)
+{
+    extern __shared__ double lds[];
+    sweep_x_body<BC, CH, POST, EPI>(a, blockIdx.y, xcd_tile(blockIdx.x, gridDim.x), W,
+                                    NW, lds);
 }
 
 template <int BC, int CH>
 __global__ __launch_bounds__(512) void sweep_y_kernel(SweepArgs a, int W, int NW)
 {
     extern __shared__ double lds[];
-    const int n = a.n, LP = line_pitch(n);
-    const size_t foff = (size_t)blockIdx.y * (size_t)a.fstride;
-    const int i0 = blockIdx.x * W;
-    load_tile_y(a.in + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
-    __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double *aux = lds + (W + wave) * LP;
-    for (int l = wave; l < W; l += NW)
-        process_line<DIR_Y, BC, CH, false>(a, lds + l * LP, aux, i0 + l, lane);
-    __syncthreads();
-    store_tile_y(a.out + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
+    sweep_y_body<BC, CH>(a, blockIdx.y, blockIdx.x, W, NW, lds);
+}
+
+template <int BC, int CH, bool POST, int EPI>
+__global__ __launch_bounds__(512) void ragged_x_kernel(const RaggedItem *items,
+                                                       const int *prefix, int nitems,
+                                                       int W, int NW)
+{
+    extern __shared__ double lds[];
+    const int b = blockIdx.x;
+    const int i = ragged_item(prefix, nitems, b);
+    const int local = b - prefix[i];
+    const int nt = items[i].ntiles;
+    const int field = local / nt;
+    // the arguments stay in global memory (scalar-cached); a private copy of
+    // the struct trips the gfx950 backend for this kernel
+    const SweepArgs &a = items[i].a;
+    sweep_x_body<BC, CH, POST, EPI>(a, field, local - field * nt, W, NW, lds);
+}
+
+template <int BC, int CH>
+__global__ __launch_bounds__(512) void ragged_y_kernel(const RaggedItem *items,
+                                                       const int *prefix, int nitems,
+                                                       int W, int NW)
+{
+    extern __shared__ double lds[];
+    const int b = blockIdx.x;
+    const int i = ragged_item(prefix, nitems, b);
+    const int local = b - prefix[i];
+    const int nt = items[i].ntiles;
+    const int field = local / nt;
+    const SweepArgs &a = items[i].a;
+    sweep_y_body<BC, CH>(a, field, local - field * nt, W, NW, lds);
 }
 
 size_t sweep_lds_bytes_generic(int n, Tile tile)
@@ -157,6 +206,71 @@ hipError_t launch_sweep_generic(Dir dir, int bc, bool post, Epi epi,
 {
     if (bc == BC_PER) return dispatch_bc<BC_PER>(dir, false, epi, a, nfields, tile, s);
     return dispatch_bc<BC_REF>(dir, post, epi, a, nfields, tile, s);
+}
+
+template <typename K>
+static hipError_t launch_r(K k, int nblocks, Tile t, size_t lds, hipStream_t s,
+                           const RaggedItem *items, const int *prefix, int nitems)
+{
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute(
+            (const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL(k, dim3(nblocks), dim3(64 * t.nw), lds, s, items, prefix,
+                       nitems, t.w, t.nw);
+    return hipGetLastError();
+}
+
+template <int BC, int CH>
+static hipError_t dispatch_r(Dir dir, bool post, Epi epi, const RaggedItem *it,
+                             const int *pf, int ni, int nb, Tile t, size_t lds,
+                             hipStream_t s)
+{
+    if (dir == DIR_Y) return launch_r(ragged_y_kernel<BC, CH>, nb, t, lds, s, it, pf, ni);
+    if (BC == BC_REF && post) {
+        if (epi == EPI_NONE)
+            return launch_r(ragged_x_kernel<BC, CH, true, EPI_NONE>, nb, t, lds, s, it, pf, ni);
+        if (epi == EPI_INIT)
+            return launch_r(ragged_x_kernel<BC, CH, true, EPI_INIT>, nb, t, lds, s, it, pf, ni);
+        return launch_r(ragged_x_kernel<BC, CH, true, EPI_CORRECT>, nb, t, lds, s, it, pf, ni);
+    }
+    if (epi == EPI_NONE)
+        return launch_r(ragged_x_kernel<BC, CH, false, EPI_NONE>, nb, t, lds, s, it, pf, ni);
+    if (epi == EPI_INIT)
+        return launch_r(ragged_x_kernel<BC, CH, false, EPI_INIT>, nb, t, lds, s, it, pf, ni);
+    return launch_r(ragged_x_kernel<BC, CH, false, EPI_CORRECT>, nb, t, lds, s, it, pf, ni);
+}
+
+template <int BC>
+static hipError_t dispatch_rbc(Dir dir, bool post, Epi epi, int ch, const RaggedItem *it,
+                               const int *pf, int ni, int nb, Tile t, size_t lds,
+                               hipStream_t s)
+{
+    switch (ch) {
+    case 1: return dispatch_r<BC, 1>(dir, post, epi, it, pf, ni, nb, t, lds, s);
+    case 2: return dispatch_r<BC, 2>(dir, post, epi, it, pf, ni, nb, t, lds, s);
+    case 4: return dispatch_r<BC, 4>(dir, post, epi, it, pf, ni, nb, t, lds, s);
+    case 8: return dispatch_r<BC, 8>(dir, post, epi, it, pf, ni, nb, t, lds, s);
+    case 16: return dispatch_r<BC, 16>(dir, post, epi, it, pf, ni, nb, t, lds, s);
+    case 32: return dispatch_r<BC, 32>(dir, post, epi, it, pf, ni, nb, t, lds, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_ragged_generic(Dir dir, int bc, bool post, Epi epi, int ch,
+                                 const RaggedItem *items, const int *prefix,
+                                 int nitems, int nblocks, Tile t, size_t lds,
+                                 hipStream_t s)
+{
+    if (nblocks <= 0 || nitems <= 0) return hipSuccess;
+    if (bc == BC_PER)
+        return dispatch_rbc<BC_PER>(dir, false, epi, ch, items, prefix, nitems, nblocks,
+                                    t, lds, s);
+    return dispatch_rbc<BC_REF>(dir, post, epi, ch, items, prefix, nitems, nblocks, t,
+                                lds, s);
 }
 
 }  // namespace ccfv

@@ -88,3 +88,27 @@ def test_null_arguments_are_errors():
     assert lib.ccfv_step(None, 0.0, 0.1) == T.CCFV_EINVAL
     assert lib.ccfv_plain_solve(None, 0.0, 1.0, 4) == T.CCFV_EINVAL
     assert b"NULL" in lib.ccfv_last_error()
+
+
+def test_batch_rejects_bad_problems():
+    with pytest.raises(ccfv.CcfvError) as ei:
+        ccfv.Batch([T.rotation_problem(64), T.rotation_problem(64).copy(nx=36)])
+    assert ei.value.code == T.CCFV_EINVAL
+    assert "multiples of 8" in str(ei.value)
+    with pytest.raises(ccfv.CcfvError) as ei:
+        ccfv.Batch([T.rotation_problem(64).copy(nu=-1.0)])
+    assert "problem 0" in str(ei.value)
+    lib = ccfv.load()
+    assert lib.ccfv_batch_create(None, 1, None) == T.CCFV_EINVAL
+    assert lib.ccfv_batch_plain(None, None, None, None) == T.CCFV_EINVAL
+    assert lib.ccfv_batch_parareal(None, None, None, None, None, 1, 0.0,
+                                   None) == T.CCFV_EINVAL
+    lib.ccfv_batch_destroy(None)
+
+
+def test_batch_without_device_fails_loudly():
+    if ccfv.device_count() > 0:
+        pytest.skip("a device is present")
+    with pytest.raises(ccfv.CcfvError) as ei:
+        ccfv.Batch([T.rotation_problem(64)])
+    assert ei.value.code == T.CCFV_EDEVICE

@@ -1,0 +1,133 @@
+"""GPU: batches of independent problems of mixed sizes, boundary modes and
+velocity fields, issued through shared ragged launches, against the same
+problems solved one at a time.  Same per-line arithmetic on the same inputs:
+every result must be bitwise equal."""
+import numpy as np
+import pytest
+
+from ccfv import types as T
+
+pytestmark = pytest.mark.gpu
+
+
+def mix():
+    """(problem, initial condition) pairs covering both sweep paths, both
+    boundary modes, every line chunk from 1 to 16 and non-square grids."""
+    g, gic = T.constant_gaussians(96, 40, 1.0, 0.5, 1e-3,
+                                  [(0.3, 0.4, 0.01, 1.0), (0.7, 0.6, 0.02, -0.5)])
+    return [
+        (T.rotation_problem(64), T.rotation_ic()),
+        (T.reference_problem(32, 0.01), T.reference_ic()),
+        (T.swirl_problem(48, T=1.0), T.cosine_bell_ic()),
+        (T.rotation_problem(128).copy(ny=64), T.rotation_ic()),
+        (g, gic),
+        (T.rotation_problem(512), T.rotation_ic()),
+        (T.rotation_problem(1024), T.rotation_ic()),
+        (T.advection_sinsin_problem(16), T.advection_sinsin_ic()),
+    ]
+
+
+def solo_plain(gpu, p, ic, t0, t1, nt):
+    with gpu.Context(p) as c:
+        c.fill_exact(ic, t0)
+        c.plain_solve(t0, t1, nt)
+        return c.get_state()
+
+
+def solo_parareal(gpu, p, ic, t0, t1, nt, ns, kmax, tol):
+    with gpu.Context(p) as c:
+        c.fill_exact(ic, t0)
+        rep = c.parareal_solve(t0, t1, nt, ns, kmax, tol)
+        return c.get_state(), rep
+
+
+def test_batch_plain_matches_solo(gpu):
+    m = mix()
+    t0 = [0.0, 0.0, 0.1, 0.0, 0.2, 0.0, 0.0, 0.0]
+    t1 = [0.25, 0.3, 0.4, 0.25, 0.5, 0.0625, 0.03125, 0.5]
+    nt = [16, 24, 9, 16, 12, 8, 4, 20]
+    with gpu.Batch([p for p, _ in m]) as b:
+        for i, (_, ic) in enumerate(m):
+            b[i].fill_exact(ic, t0[i])
+        b.plain(t0, t1, nt)
+        got = [b[i].get_state() for i in range(len(m))]
+    for i, (p, ic) in enumerate(m):
+        ref = solo_plain(gpu, p, ic, t0[i], t1[i], nt[i])
+        assert np.array_equal(got[i], ref), i
+
+
+@pytest.mark.parametrize("kmax,tol", [(3, 0.0), (8, 1e-9)])
+def test_batch_parareal_matches_solo(gpu, kmax, tol):
+    m = mix()[:5] + [mix()[7]]
+    nt = [64, 48, 36, 32, 40, 32]
+    ns = [8, 6, 4, 8, 5, 8]
+    t1 = [0.5, 0.25, 0.6, 0.25, 0.5, 0.5]
+    with gpu.Batch([p for p, _ in m]) as b:
+        for i, (_, ic) in enumerate(m):
+            b[i].fill_exact(ic, 0.0)
+        reps = b.parareal(0.0, t1, nt, ns, kmax, tol)
+        got = [b[i].get_state() for i in range(len(m))]
+    for i, (p, ic) in enumerate(m):
+        ref, rr = solo_parareal(gpu, p, ic, 0.0, t1[i], nt[i], ns[i], kmax, tol)
+        assert np.array_equal(got[i], ref), i
+        assert reps[i].iterations == rr.iterations, i
+        assert reps[i].converged == rr.converged, i
+        # the batch runs no speculative fine work: exactly the slices that
+        # can still change, (ns - k) * J field-steps in iteration k
+        J = nt[i] // ns[i]
+        its = reps[i].iterations
+        assert reps[i].fine_steps_run == sum((ns[i] - k) * J for k in range(its)), i
+        assert reps[i].coarse_steps_run == rr.coarse_steps_run, i
+        np.testing.assert_allclose(reps[i].increments, rr.increments, rtol=1e-12)
+
+
+def test_batch_parareal_large_mix(gpu):
+    """512^2 and 1024^2 rotation beside small grids: the chunk-8/16 fast
+    path and the chunk-1 generic path in the same rounds."""
+    m = [mix()[i] for i in (5, 6, 1, 7)]
+    nt, ns = [32, 16, 16, 16], [8, 4, 4, 8]
+    t1 = [0.125, 0.0625, 0.1, 0.25]
+    with gpu.Batch([p for p, _ in m]) as b:
+        for i, (_, ic) in enumerate(m):
+            b[i].fill_exact(ic, 0.0)
+        reps = b.parareal(0.0, t1, nt, ns, 2, 0.0)
+        got = [b[i].get_state() for i in range(len(m))]
+    for i, (p, ic) in enumerate(m):
+        ref, rr = solo_parareal(gpu, p, ic, 0.0, t1[i], nt[i], ns[i], 2, 0.0)
+        assert np.array_equal(got[i], ref), i
+        np.testing.assert_allclose(reps[i].increments, rr.increments, rtol=1e-12)
+
+
+def test_batch_repeated_and_kmax_beyond_slices(gpu):
+    """Two solves on one batch (descriptor arenas reused) and kmax above the
+    slice count: min(kmax, n_slices) iterations, converged, the serial fine
+    solution."""
+    m = [mix()[0], mix()[1]]
+    with gpu.Batch([p for p, _ in m]) as b:
+        for rep_i in range(2):
+            for i, (_, ic) in enumerate(m):
+                b[i].fill_exact(ic, 0.0)
+            reps = b.parareal(0.0, 0.25, 32, 4, 6, 0.0)
+            for i, (p, ic) in enumerate(m):
+                assert reps[i].iterations == 4 and reps[i].converged == 1
+                ref = solo_plain(gpu, p, ic, 0.0, 0.25, 32)
+                rel = np.linalg.norm(b[i].get_state() - ref) / np.linalg.norm(ref)
+                assert rel <= 1e-12, (rep_i, i)
+
+
+def test_solo_kmax_beyond_slices(gpu):
+    p, ic = mix()[0]
+    u, rep = solo_parareal(gpu, p, ic, 0.0, 0.25, 32, 4, 9, 1e-30)
+    assert rep.iterations == 4 and rep.converged == 1
+    ref = solo_plain(gpu, p, ic, 0.0, 0.25, 32)
+    assert np.linalg.norm(u - ref) / np.linalg.norm(ref) <= 1e-12
+
+
+def test_batch_not_converged_status(gpu):
+    m = [mix()[0], mix()[2]]
+    with gpu.Batch([p for p, _ in m]) as b:
+        for i, (_, ic) in enumerate(m):
+            b[i].fill_exact(ic, 0.0)
+        with pytest.raises(gpu.CcfvError) as ei:
+            b.parareal(0.0, 0.5, 64, 8, 2, 1e-15, raise_on_notconv=True)
+        assert ei.value.code == T.CCFV_ENOTCONV
