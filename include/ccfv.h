@@ -287,6 +287,60 @@ int  ccfv_batch_parareal(ccfv_batch *b, const double *t0, const double *t1,
                          const int32_t *nt_fine, const int32_t *n_slices,
                          int32_t kmax, double tol, ccfv_parareal_report *reports);
 
+/* ---- serving engine ------------------------------------------------------
+ * (no counterpart in the reference, which solves one problem per process.)
+ * Parareal solve requests of mixed sizes, admitted by a device-memory budget
+ * and advanced together: continuous batching.  The engine allocates its
+ * arena once (mem_budget bytes).  A request waits in arrival order until an
+ * arena block of ccfv_request_bytes() and one of max_active slots are free;
+ * a request that does not fit is passed over by later ones at most
+ * max_bypass times.  Its whole Parareal state then stays resident until it
+ * finishes.  Each ccfv_server_step admits what fits and advances every
+ * resident request by one Parareal iteration (a new one also runs its
+ * k = 0 coarse sweep) through shared ragged launches.  A request's result
+ * and report are bitwise those of ccfv_parareal_solve on the same problem
+ * alone.  One engine is driven by one host thread. */
+typedef struct ccfv_server ccfv_server;
+typedef struct ccfv_server_config {
+    uint64_t mem_budget;   /* arena bytes (0: half of the free device memory) */
+    int32_t  max_active;   /* resident requests, 1..4096 (0: 256)             */
+    int32_t  max_slices;   /* largest n_slices accepted, 1..1024 (0: 64)      */
+    int32_t  max_bypass;   /* see above (0: 8)                                */
+    int32_t  reserved;
+} ccfv_server_config;
+
+typedef struct ccfv_request {
+    ccfv_problem problem;  /* grids: multiples of 8 cells                      */
+    ccfv_ic  ic;           /* initial data at t0 when submitted without u0    */
+    double   t0, t1;
+    int32_t  nt_fine, n_slices, kmax, reserved;
+    double   tol;          /* as ccfv_parareal_solve                          */
+} ccfv_request;
+
+typedef struct ccfv_server_status {
+    int32_t  active, queued, finished, reserved;  /* finished: not yet taken */
+    uint64_t mem_used, mem_budget;
+    uint64_t steps;
+    double   last_step_ms;  /* device time of the last step                  */
+} ccfv_server_status;
+
+/* device bytes a request holds while resident */
+int  ccfv_request_bytes(const ccfv_request *r, uint64_t *bytes);
+int  ccfv_server_create(const ccfv_server_config *cfg, ccfv_server **out);
+void ccfv_server_destroy(ccfv_server *s);
+/* queues a request (u0: nx*ny initial cell averages, or NULL for r->ic);
+ * CCFV_EBUSY if it can never fit the budget.  *id identifies it. */
+int  ccfv_server_submit(ccfv_server *s, const ccfv_request *r, const double *u0,
+                        int64_t *id);
+/* admission + one iteration of every resident request; blocks until done */
+int  ccfv_server_step(ccfv_server *s, ccfv_server_status *st);
+/* oldest finished request not yet returned here, or *id = -1 */
+int  ccfv_server_next_done(ccfv_server *s, int64_t *id);
+/* result of a finished request (u: nx*ny, may be NULL; rep may be NULL;
+ * *status: CCFV_OK, CCFV_ENOTCONV, CCFV_ECFL or CCFV_ENONFINITE); frees it */
+int  ccfv_server_result(ccfv_server *s, int64_t id, double *u,
+                        ccfv_parareal_report *rep, int32_t *status);
+
 #ifdef __cplusplus
 }
 #endif

@@ -89,6 +89,20 @@ _SIGS = {
                                            ctypes.POINTER(ctypes.c_int32),
                                            ctypes.c_int32, ctypes.c_double,
                                            ctypes.POINTER(ParaReport)]),
+    "ccfv_request_bytes": (ctypes.c_int, [ctypes.POINTER(types.Request),
+                                          ctypes.POINTER(ctypes.c_uint64)]),
+    "ccfv_server_create": (ctypes.c_int, [ctypes.POINTER(types.ServerConfig),
+                                          ctypes.POINTER(ctypes.c_void_p)]),
+    "ccfv_server_destroy": (None, [ctypes.c_void_p]),
+    "ccfv_server_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(types.Request),
+                                          _dp, ctypes.POINTER(ctypes.c_int64)]),
+    "ccfv_server_step": (ctypes.c_int, [ctypes.c_void_p,
+                                        ctypes.POINTER(types.ServerStatus)]),
+    "ccfv_server_next_done": (ctypes.c_int, [ctypes.c_void_p,
+                                             ctypes.POINTER(ctypes.c_int64)]),
+    "ccfv_server_result": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _dp,
+                                          ctypes.POINTER(ParaReport),
+                                          ctypes.POINTER(ctypes.c_int32)]),
 }
 
 _lib = None
@@ -363,5 +377,79 @@ class Batch:
         return list(reps)
 
 
-__all__ = ["Context", "Batch", "CcfvError", "load", "device_count", "types",
-           "Problem", "IC", "ParaReport", "exported_symbols"]
+def request_bytes(req: "types.Request") -> int:
+    """Device bytes a request holds while resident in a Server."""
+    b = ctypes.c_uint64()
+    _check(load().ccfv_request_bytes(ctypes.byref(req), ctypes.byref(b)),
+           "ccfv_request_bytes")
+    return b.value
+
+
+class Server:
+    """The serving engine (ccfv_server_*): continuous batching of Parareal
+    solve requests under a device-memory budget."""
+
+    def __init__(self, mem_budget: int = 0, max_active: int = 0,
+                 max_slices: int = 0, max_bypass: int = 0):
+        cfg = types.ServerConfig(mem_budget, max_active, max_slices, max_bypass, 0)
+        h = ctypes.c_void_p()
+        _check(load().ccfv_server_create(ctypes.byref(cfg), ctypes.byref(h)),
+               "ccfv_server_create")
+        self._h = h
+        self._shape = {}
+
+    def close(self):
+        if self._h:
+            load().ccfv_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, req: "types.Request", u0: np.ndarray | None = None) -> int:
+        rid = ctypes.c_int64()
+        p = None
+        if u0 is not None:
+            a = np.ascontiguousarray(u0, dtype=np.float64).reshape(-1)
+            if a.size != req.problem.nx * req.problem.ny:
+                raise ValueError("u0 has the wrong size")
+            p = _ptr(a)
+        _check(load().ccfv_server_submit(self._h, ctypes.byref(req), p,
+                                         ctypes.byref(rid)), "ccfv_server_submit")
+        self._shape[rid.value] = req.problem.nx * req.problem.ny
+        return rid.value
+
+    def step(self) -> "types.ServerStatus":
+        st = types.ServerStatus()
+        _check(load().ccfv_server_step(self._h, ctypes.byref(st)), "ccfv_server_step")
+        return st
+
+    def next_done(self) -> int:
+        rid = ctypes.c_int64()
+        _check(load().ccfv_server_next_done(self._h, ctypes.byref(rid)),
+               "ccfv_server_next_done")
+        return rid.value
+
+    def result(self, rid: int):
+        """(state, report, status) of a finished request; releases it."""
+        out = np.empty(self._shape[rid], dtype=np.float64)
+        rep = ParaReport()
+        st = ctypes.c_int32()
+        _check(load().ccfv_server_result(self._h, rid, _ptr(out), ctypes.byref(rep),
+                                         ctypes.byref(st)), "ccfv_server_result")
+        del self._shape[rid]
+        return out, rep, st.value
+
+
+__all__ = ["Context", "Batch", "Server", "CcfvError", "load", "device_count",
+           "types", "Problem", "IC", "ParaReport", "exported_symbols",
+           "request_bytes"]

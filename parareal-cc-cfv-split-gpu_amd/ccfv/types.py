@@ -114,6 +114,40 @@ class ParaReport(ctypes.Structure):
         return [self.increment[i] for i in range(self.iterations)]
 
 
+class ServerConfig(ctypes.Structure):
+    """ccfv_server_config."""
+    _fields_ = [("mem_budget", ctypes.c_uint64), ("max_active", ctypes.c_int32),
+                ("max_slices", ctypes.c_int32), ("max_bypass", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class Request(ctypes.Structure):
+    """ccfv_request: one Parareal solve for the serving engine."""
+    _fields_ = [("problem", Problem), ("ic", IC), ("t0", ctypes.c_double),
+                ("t1", ctypes.c_double), ("nt_fine", ctypes.c_int32),
+                ("n_slices", ctypes.c_int32), ("kmax", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("tol", ctypes.c_double)]
+
+
+class ServerStatus(ctypes.Structure):
+    """ccfv_server_status."""
+    _fields_ = [("active", ctypes.c_int32), ("queued", ctypes.c_int32),
+                ("finished", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("mem_used", ctypes.c_uint64), ("mem_budget", ctypes.c_uint64),
+                ("steps", ctypes.c_uint64), ("last_step_ms", ctypes.c_double)]
+
+
+def make_request(problem: Problem, ic: IC, t0: float, t1: float, nt_fine: int,
+                 n_slices: int, kmax: int, tol: float = 0.0) -> Request:
+    r = Request()
+    r.problem = problem.copy()
+    r.ic = ic
+    r.t0, r.t1 = t0, t1
+    r.nt_fine, r.n_slices, r.kmax = nt_fine, n_slices, kmax
+    r.tol = tol
+    return r
+
+
 KSTAT_NAMES = ["x_batch", "y_batch", "x_single", "y_single", "x_epilog"]
 
 

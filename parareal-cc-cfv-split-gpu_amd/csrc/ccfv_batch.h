@@ -1,5 +1,6 @@
-// ccfv_batch.h — batch of independent problems (ccfv_batch.hip) and the
-// phase builder shared with the serving loop.
+// ccfv_batch.h — many problems through shared ragged launches: the phase
+// builder and runner (ccfv_phase.hip), the per-problem Parareal job used by
+// the batch (ccfv_batch.hip) and the serving engine (ccfv_server.hip).
 #pragma once
 
 #include <map>
@@ -24,7 +25,7 @@ struct StageItem {
 // one ragged launch of a phase; items / prefix are byte offsets into the
 // phase's descriptor arena
 struct Launch {
-    enum Kind { SWEEP, CORR, REDUCE } kind;
+    enum Kind { SWEEP, CORR, REDUCE, COPY } kind;
     Dir dir = DIR_X;
     Tile tile{0, 0};
     int ch = 1, bc = 0;
@@ -44,20 +45,83 @@ struct Phase {
     void add_sweep_stage(std::vector<StageItem> &items);
     void add_corrections(const std::vector<RaggedCorr> &c, const std::vector<int> &blocks);
     void add_reductions(const std::vector<RaggedReduce> &r);
+    void add_copies(const std::vector<RaggedCopy> &c);
+    bool empty() const { return launches.empty(); }
 };
+
+// Issues phases on one stream.  The descriptors go through two pinned host +
+// device arenas used alternately, so building phase p+1 on the host overlaps
+// the device running phase p.
+struct PhaseRunner {
+    hipStream_t stream = nullptr;
+    char *hbuf[2] = {nullptr, nullptr};
+    char *dbuf[2] = {nullptr, nullptr};
+    size_t cap[2] = {0, 0};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    int slot = 0;
+    int init();
+    void release();
+    int run(Phase &ph);
+};
+
+// the three stages of one Strang step of one problem
+struct StepSpec {
+    ccfv_ctx *c;
+    const SolveTable *tx, *ty;
+    const double *in;
+    double *out;
+    int nf;
+    double t0, tDT, jdt, h;
+    int field0;
+    Epi epi;
+    double *g_cache;
+    const double *f_in;
+    double *u_next;
+    double *partial;
+    int shape;  // 1: batch launch shapes (fine propagators), 0: one-field
+};
+void add_step_items(const StepSpec &s, std::vector<StageItem> st[3]);
+void flush_round(Phase &ph, std::vector<StageItem> st[3]);
+
+// launch shapes every grid of a batch can use (multiples of 8 cells)
+void shared_tiles(ccfv_ctx *c);
+
+// One problem's Parareal run, advanced one iteration per phase.  The work
+// and its order are those of ccfv_parareal_solve: the k = 0 coarse sweep
+// (cached G), then per iteration the fine propagators of slices k..ns-1,
+// the correction of slot k+1, the coarse chain with the fused correction and
+// the per-slot increment sums.
+struct ParaJob {
+    ccfv_ctx *c = nullptr;
+    int ns = 0, J = 0, nb = 0, kmax = 0;
+    double t0 = 0, DT = 0, dt = 0, tol = 0;
+    const SolveTable *fx = nullptr, *fy = nullptr, *cx = nullptr, *cy = nullptr;
+    int k = 0;            // next iteration
+    bool started = false; // k = 0 coarse sweep issued
+    bool running = true;
+    int status = CCFV_OK;
+    size_t sums_off = 0;  // this phase's slot sums in the shared sums buffer
+    std::vector<double> slot_u2;
+    ccfv_parareal_report R{};
+};
+// validates the plan, sets up the tables and the report (c->U etc. must be
+// in place: ensure_para or the caller's own buffers)
+int job_setup(ParaJob &j, ccfv_ctx *c, double t0, double t1, int nt_fine, int ns,
+              int kmax, double tol);
+// Appends, for every job, its k = 0 coarse sweep if not yet issued and then
+// (if it has iterations left) one iteration.  Jobs advance in lock-step
+// rounds; slot sums go to sums_dev at each job's sums_off (2 (ns+1) doubles),
+// assigned here.  Returns the number of doubles of sums_dev used.
+size_t append_iteration(Phase &ph, std::vector<ParaJob *> &jobs, double *sums_dev);
+// reads an iteration's slot sums (host copy of sums_dev) into the job
+void finish_iteration(ParaJob &j, const double *sums_host);
 
 }  // namespace rt
 }  // namespace ccfv
 
 struct ccfv_batch {
     std::vector<ccfv_ctx *> ctx;
-    hipStream_t stream = nullptr;
-    // two descriptor arenas (pinned host + device), alternating by phase
-    char *hbuf[2] = {nullptr, nullptr};
-    char *dbuf[2] = {nullptr, nullptr};
-    size_t cap[2] = {0, 0};
-    hipEvent_t arena_ev[2] = {nullptr, nullptr};
-    int slot = 0;
+    ccfv::rt::PhaseRunner run;
     // per-slot increment sums of every problem
     double *sums_dev = nullptr, *sums_host = nullptr;
     size_t sums_cap = 0;

@@ -121,6 +121,14 @@ struct RaggedReduce {
 };
 hipError_t launch_ragged_reduce(const RaggedReduce *items, const int *prefix,
                                 int nitems, int nblocks, hipStream_t s);
+// copies of many fields: item i copies n doubles src -> dst
+struct RaggedCopy {
+    const double *src;
+    double *dst;
+    size_t n;
+};
+hipError_t launch_ragged_copy(const RaggedCopy *items, const int *prefix, int nitems,
+                              int nblocks, hipStream_t s);
 // element-wise kernels (ccfv_misc.hip)
 hipError_t launch_correction(const double *g_new, const double *g_old,
                              const double *f, double *u, size_t n,

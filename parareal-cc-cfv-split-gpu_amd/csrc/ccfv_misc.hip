@@ -178,6 +178,28 @@ hipError_t launch_ragged_reduce(const RaggedReduce *items, const int *prefix,
     return hipGetLastError();
 }
 
+// ragged copies: block b copies element blocks of item i
+__global__ __launch_bounds__(256) void ragged_copy_kernel(const RaggedCopy *items,
+                                                          const int *prefix, int nitems)
+{
+    const int b = blockIdx.x;
+    const int i = __builtin_amdgcn_readfirstlane(ragged_item(prefix, nitems, b));
+    const RaggedCopy it = items[i];
+    const int nb = prefix[i + 1] - prefix[i], lb = b - prefix[i];
+    for (size_t e = lb * (size_t)blockDim.x + threadIdx.x; e < it.n;
+         e += (size_t)nb * blockDim.x)
+        it.dst[e] = it.src[e];
+}
+
+hipError_t launch_ragged_copy(const RaggedCopy *items, const int *prefix, int nitems,
+                              int nblocks, hipStream_t s)
+{
+    if (nblocks <= 0 || nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ragged_copy_kernel, dim3(nblocks), dim3(256), 0, s, items, prefix,
+                       nitems);
+    return hipGetLastError();
+}
+
 __global__ void copy_kernel(const double *src, double *dst, size_t n)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;

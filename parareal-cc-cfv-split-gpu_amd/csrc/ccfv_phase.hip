@@ -1,0 +1,445 @@
+// ccfv_phase.hip — phases of ragged launches over many problems, and the
+// per-problem Parareal job that the batch and the serving engine advance.
+//
+// A phase is a list of launches whose work-list descriptors (one RaggedItem,
+// RaggedCorr, RaggedReduce or RaggedCopy per problem, plus a block prefix per
+// launch) are built on the host and uploaded with one copy.  Problems advance
+// in lock-step rounds: in round r every problem that has an r-th sub-step
+// contributes its blocks to one launch per sweep stage and line class.
+// Every problem sees exactly the kernels, inputs and order it gets when
+// solved alone, so its result is bitwise the same.
+#include "ccfv_batch.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace ccfv {
+namespace rt {
+
+void shared_tiles(ccfv_ctx *c)
+{
+    const bool post = post_flag(c);
+    const Tile x1{2, 2}, y1{2, 2}, yb{2, 1};
+    Tile xb{8, 4};
+    if (!tile_ok(xb, c->P.nx, c->P.ny, DIR_X, post)) xb = Tile{2, 2};
+    c->tile[DIR_X][0] = x1;
+    c->tile[DIR_X][1] = xb;
+    c->tile[DIR_Y][0] = y1;
+    c->tile[DIR_Y][1] = yb;
+    c->tiles_version++;
+    clear_graphs(c);
+}
+
+// ---------------------------------------------------------------------------
+// phase builder
+// ---------------------------------------------------------------------------
+void Phase::clear()
+{
+    host.clear();
+    launches.clear();
+}
+
+size_t Phase::append(const void *p, size_t bytes)
+{
+    size_t off = (host.size() + 15) & ~(size_t)15;
+    host.resize(off + bytes);
+    memcpy(host.data() + off, p, bytes);
+    return off;
+}
+
+// sweep items of one stage, grouped into launches
+void Phase::add_sweep_stage(std::vector<StageItem> &items)
+{
+    // one launch shares the kernel instance (line chunk, boundary mode, path,
+    // step-3 variant, epilogue) and the block shape
+    struct Key {
+        int ch, bc, fast, post, epi, w, nw;
+        bool operator<(const Key &o) const
+        {
+            return std::tie(ch, bc, fast, post, epi, w, nw) <
+                   std::tie(o.ch, o.bc, o.fast, o.post, o.epi, o.w, o.nw);
+        }
+    };
+    std::map<Key, std::vector<const StageItem *>> groups;
+    for (const StageItem &s : items) {
+        Key k{chunk_for(s.item.a.n), s.bc, s.item.a.lineS ? 1 : 0, s.post ? 1 : 0,
+              (int)s.epi, s.tile.w, s.tile.nw};
+        groups[k].push_back(&s);
+    }
+    for (auto &kv : groups) {
+        const Key &k = kv.first;
+        std::vector<RaggedItem> it;
+        std::vector<int> prefix(1, 0);
+        size_t lds = 0;
+        for (const StageItem *s : kv.second) {
+            it.push_back(s->item);
+            prefix.push_back(prefix.back() + s->item.nfields * s->item.ntiles);
+            size_t b = k.fast ? sweep_lds_bytes_fast(s->item.a.n, s->tile)
+                              : sweep_lds_bytes_generic(s->item.a.n, s->tile);
+            lds = std::max(lds, b);
+        }
+        Launch L;
+        L.kind = Launch::SWEEP;
+        L.dir = kv.second.front()->dir;
+        L.tile = kv.second.front()->tile;
+        L.ch = k.ch;
+        L.bc = k.bc;
+        L.fast = k.fast != 0;
+        L.post = k.post != 0;
+        L.epi = (Epi)k.epi;
+        L.nitems = (int)it.size();
+        L.nblocks = prefix.back();
+        L.lds = lds;
+        L.items = append(it.data(), sizeof(RaggedItem) * it.size());
+        L.prefix = append(prefix.data(), sizeof(int) * prefix.size());
+        launches.push_back(L);
+    }
+}
+
+void Phase::add_corrections(const std::vector<RaggedCorr> &c, const std::vector<int> &blocks)
+{
+    if (c.empty()) return;
+    std::vector<int> prefix(1, 0);
+    for (int b : blocks) prefix.push_back(prefix.back() + b);
+    Launch L;
+    L.kind = Launch::CORR;
+    L.nitems = (int)c.size();
+    L.nblocks = prefix.back();
+    L.items = append(c.data(), sizeof(RaggedCorr) * c.size());
+    L.prefix = append(prefix.data(), sizeof(int) * prefix.size());
+    launches.push_back(L);
+}
+
+void Phase::add_reductions(const std::vector<RaggedReduce> &r)
+{
+    if (r.empty()) return;
+    std::vector<int> prefix(1, 0);
+    for (const RaggedReduce &x : r) prefix.push_back(prefix.back() + x.count);
+    Launch L;
+    L.kind = Launch::REDUCE;
+    L.nitems = (int)r.size();
+    L.nblocks = prefix.back();
+    L.items = append(r.data(), sizeof(RaggedReduce) * r.size());
+    L.prefix = append(prefix.data(), sizeof(int) * prefix.size());
+    launches.push_back(L);
+}
+
+void Phase::add_copies(const std::vector<RaggedCopy> &c)
+{
+    if (c.empty()) return;
+    std::vector<int> prefix(1, 0);
+    for (const RaggedCopy &x : c) {
+        size_t b = (x.n + 255) / 256;
+        prefix.push_back(prefix.back() + (int)std::min<size_t>(std::max<size_t>(b, 1), 1024));
+    }
+    Launch L;
+    L.kind = Launch::COPY;
+    L.nitems = (int)c.size();
+    L.nblocks = prefix.back();
+    L.items = append(c.data(), sizeof(RaggedCopy) * c.size());
+    L.prefix = append(prefix.data(), sizeof(int) * prefix.size());
+    launches.push_back(L);
+}
+
+// ---------------------------------------------------------------------------
+// phase runner
+// ---------------------------------------------------------------------------
+int PhaseRunner::init()
+{
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
+    return CCFV_OK;
+}
+
+void PhaseRunner::release()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (int s = 0; s < 2; s++) {
+        if (hbuf[s]) (void)hipHostFree(hbuf[s]);
+        if (dbuf[s]) (void)hipFree(dbuf[s]);
+        if (done[s]) (void)hipEventDestroy(done[s]);
+        hbuf[s] = dbuf[s] = nullptr;
+        done[s] = nullptr;
+        cap[s] = 0;
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+}
+
+int PhaseRunner::run(Phase &ph)
+{
+    if (ph.launches.empty()) return CCFV_OK;
+    const size_t bytes = ph.host.size();
+    const int s = slot;
+    slot ^= 1;
+    // this arena was last read by the phase before the previous one
+    HIPCHK(hipEventSynchronize(done[s]));
+    if (bytes > cap[s]) {
+        if (hbuf[s]) (void)hipHostFree(hbuf[s]);
+        if (dbuf[s]) (void)hipFree(dbuf[s]);
+        hbuf[s] = nullptr;
+        dbuf[s] = nullptr;
+        cap[s] = 0;
+        const size_t c = std::max(bytes, (size_t)1 << 16);
+        HIPCHK(hipHostMalloc((void **)&hbuf[s], c));
+        HIPCHK(hipMalloc((void **)&dbuf[s], c));
+        cap[s] = c;
+    }
+    memcpy(hbuf[s], ph.host.data(), bytes);
+    HIPCHK(hipMemcpyAsync(dbuf[s], hbuf[s], bytes, hipMemcpyHostToDevice, stream));
+    const char *d = dbuf[s];
+    for (const Launch &L : ph.launches) {
+        hipError_t e = hipSuccess;
+        const int *pf = (const int *)(d + L.prefix);
+        switch (L.kind) {
+        case Launch::SWEEP: {
+            const RaggedItem *it = (const RaggedItem *)(d + L.items);
+            e = L.fast ? launch_ragged_fast(L.dir, L.epi, L.ch, it, pf, L.nitems, L.nblocks,
+                                            L.tile, L.lds, stream)
+                       : launch_ragged_generic(L.dir, L.bc, L.post, L.epi, L.ch, it, pf,
+                                               L.nitems, L.nblocks, L.tile, L.lds, stream);
+            break;
+        }
+        case Launch::CORR:
+            e = launch_ragged_correction((const RaggedCorr *)(d + L.items), pf, L.nitems,
+                                         L.nblocks, stream);
+            break;
+        case Launch::REDUCE:
+            e = launch_ragged_reduce((const RaggedReduce *)(d + L.items), pf, L.nitems,
+                                     L.nblocks, stream);
+            break;
+        case Launch::COPY:
+            e = launch_ragged_copy((const RaggedCopy *)(d + L.items), pf, L.nitems,
+                                   L.nblocks, stream);
+            break;
+        }
+        if (e != hipSuccess) return hip_fail(e, "ragged launch");
+    }
+    HIPCHK(hipEventRecord(done[s], stream));
+    return CCFV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Strang steps as stage items
+// ---------------------------------------------------------------------------
+void add_step_items(const StepSpec &s, std::vector<StageItem> st[3])
+{
+    ccfv_ctx *c = s.c;
+    // fine propagators use the batch shapes; the coarse chain and the plain
+    // sweep the one-field shapes, which also fix the epilogue's partial sums
+    const int bt = s.shape;
+    const long long fs = (long long)c->ncell;
+    StageItem a;
+    a.dir = DIR_X;
+    a.bc = c->P.bc;
+    a.post = false;
+    a.epi = EPI_NONE;
+    a.tile = c->tile[DIR_X][bt];
+    a.item.a = make_args(c, DIR_X, 1, *s.tx, s.h, s.t0, s.tDT, s.field0, s.jdt);
+    a.item.a.in = s.in;
+    a.item.a.out = s.out;
+    a.item.a.fstride = fs;
+    a.item.nfields = s.nf;
+    a.item.ntiles = c->P.ny / a.tile.w;
+    st[0].push_back(a);
+    StageItem b;
+    b.dir = DIR_Y;
+    b.bc = c->P.bc;
+    b.post = false;
+    b.epi = EPI_NONE;
+    b.tile = c->tile[DIR_Y][bt];
+    b.item.a = make_args(c, DIR_Y, 2, *s.ty, s.h, s.t0, s.tDT, s.field0, s.jdt);
+    b.item.a.in = s.out;
+    b.item.a.out = s.out;
+    b.item.a.fstride = fs;
+    b.item.nfields = s.nf;
+    b.item.ntiles = c->P.nx / b.tile.w;
+    st[1].push_back(b);
+    StageItem x;
+    x.dir = DIR_X;
+    x.bc = c->P.bc;
+    x.post = post_flag(c);
+    x.epi = s.epi;
+    x.tile = c->tile[DIR_X][bt];
+    x.item.a = make_args(c, DIR_X, 3, *s.tx, s.h, s.t0, s.tDT, s.field0, s.jdt);
+    x.item.a.in = s.out;
+    x.item.a.out = s.out;
+    x.item.a.fstride = fs;
+    x.item.a.g_cache = s.g_cache;
+    x.item.a.f_in = s.f_in;
+    x.item.a.u_next = s.u_next;
+    x.item.a.partial = s.partial;
+    x.item.nfields = s.nf;
+    x.item.ntiles = c->P.ny / x.tile.w;
+    st[2].push_back(x);
+}
+
+void flush_round(Phase &ph, std::vector<StageItem> st[3])
+{
+    for (int i = 0; i < 3; i++) {
+        if (!st[i].empty()) ph.add_sweep_stage(st[i]);
+        st[i].clear();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Parareal jobs
+// ---------------------------------------------------------------------------
+int job_setup(ParaJob &j, ccfv_ctx *c, double t0, double t1, int nt_fine, int ns,
+              int kmax, double tol)
+{
+    if (ns < 1 || nt_fine < ns || nt_fine % ns)
+        return fail(CCFV_EINVAL, "nt_fine must be a positive multiple of n_slices");
+    if (kmax < 0 || kmax > CCFV_MAX_ITERS) return fail(CCFV_EINVAL, "kmax must be in [0, 64]");
+    if (!(t1 > t0)) return fail(CCFV_EINVAL, "need t1 > t0");
+    j = ParaJob();
+    j.c = c;
+    j.ns = ns;
+    j.J = nt_fine / ns;
+    j.t0 = t0;
+    j.DT = (t1 - t0) / ns;
+    j.dt = j.DT / (double)j.J;
+    j.nb = c->chain_blocks;
+    // iteration k fixes slot k+1; past n_slices iterations nothing changes
+    j.kmax = std::min(kmax, ns);
+    j.tol = tol;
+    int rc;
+    if ((rc = get_table(c, DIR_X, j.dt, &j.fx)) || (rc = get_table(c, DIR_Y, j.dt, &j.fy)) ||
+        (rc = get_table(c, DIR_X, j.DT, &j.cx)) || (rc = get_table(c, DIR_Y, j.DT, &j.cy)))
+        return rc;
+    j.slot_u2.assign(ns + 1, 0.0);
+    memset(&j.R, 0, sizeof(j.R));
+    j.R.n_slices = ns;
+    j.R.j_fine = j.J;
+    j.k = 0;
+    j.started = false;
+    j.running = j.kmax > 0;
+    j.status = CCFV_OK;
+    return CCFV_OK;
+}
+
+namespace {
+
+StepSpec coarse_spec(const ParaJob &p, int n, Epi epi)
+{
+    const size_t N = p.c->ncell;
+    StepSpec s{p.c, p.cx, p.cy, p.c->U + n * N, p.c->Wk, 1, p.t0, p.DT, 0.0, p.DT, n, epi,
+               p.c->G + n * N, epi == EPI_CORRECT ? p.c->F + n * N : nullptr,
+               p.c->U + (n + 1) * N,
+               epi == EPI_CORRECT ? p.c->partial + 2 * (size_t)p.nb * (n + 1) : nullptr, 0};
+    return s;
+}
+
+}  // namespace
+
+size_t append_iteration(Phase &ph, std::vector<ParaJob *> &jobs, double *sums_dev)
+{
+    std::vector<StageItem> st[3];
+    // k = 0 coarse sweep: U[n+1] = G(U[n]), G cached (T2S4_Parareal.cu:1264-1280)
+    int rounds = 0;
+    for (ParaJob *p : jobs)
+        if (!p->started) rounds = std::max(rounds, p->ns);
+    for (int n = 0; n < rounds; n++) {
+        for (ParaJob *p : jobs)
+            if (!p->started && n < p->ns) add_step_items(coarse_spec(*p, n, EPI_INIT), st);
+        flush_round(ph, st);
+    }
+    for (ParaJob *p : jobs)
+        if (!p->started) {
+            p->started = true;
+            p->R.coarse_steps_run = p->ns;
+        }
+    // iterating jobs this phase
+    std::vector<ParaJob *> it;
+    for (ParaJob *p : jobs)
+        if (p->running && p->k < p->kmax) it.push_back(p);
+    // fine propagators of slices k..ns-1, all slices of a job as one batch
+    // of fields (T2S4_Parareal.cu:1319-1324)
+    int Jmax = 0;
+    for (ParaJob *p : it) Jmax = std::max(Jmax, p->J);
+    for (int jj = 0; jj < Jmax; jj++) {
+        for (ParaJob *p : it) {
+            if (jj >= p->J) continue;
+            const size_t N = p->c->ncell;
+            const double *src = jj == 0 ? p->c->U + p->k * N : p->c->F + p->k * N;
+            StepSpec s{p->c, p->fx, p->fy, src, p->c->F + p->k * N, p->ns - p->k, p->t0,
+                       p->DT, jj * p->dt, p->dt, p->k, EPI_NONE, nullptr, nullptr, nullptr,
+                       nullptr, 1};
+            add_step_items(s, st);
+        }
+        flush_round(ph, st);
+    }
+    // slot k+1 by the correction alone (T2S4_Parareal.cu:1327-1330)
+    std::vector<RaggedCorr> corr;
+    std::vector<int> cblocks;
+    for (ParaJob *p : it) {
+        const size_t N = p->c->ncell;
+        const int k = p->k;
+        corr.push_back(RaggedCorr{p->c->G + k * N, p->c->G + k * N, p->c->F + k * N,
+                                  p->c->U + (k + 1) * N, N,
+                                  p->c->partial + 2 * (size_t)p->nb * (k + 1)});
+        cblocks.push_back(p->nb);
+    }
+    ph.add_corrections(corr, cblocks);
+    // coarse chain from slice k+1 with the correction fused into its last
+    // sweep; round r is step k+1+r of every job
+    int crounds = 0;
+    for (ParaJob *p : it) crounds = std::max(crounds, p->ns - p->k - 1);
+    for (int r = 0; r < crounds; r++) {
+        for (ParaJob *p : it) {
+            const int n = p->k + 1 + r;
+            if (n < p->ns) add_step_items(coarse_spec(*p, n, EPI_CORRECT), st);
+        }
+        flush_round(ph, st);
+    }
+    // per-slot increment sums of slots k+1..ns
+    std::vector<RaggedReduce> red;
+    size_t used = 0;
+    for (ParaJob *p : it) {
+        p->sums_off = used;
+        used += 2 * (size_t)(p->ns + 1);
+        red.push_back(RaggedReduce{p->c->partial, sums_dev + p->sums_off, p->nb, p->k + 1,
+                                   p->ns - p->k});
+        p->R.fine_steps_run += (double)(p->ns - p->k) * p->J;
+        p->R.coarse_steps_run += p->ns - p->k - 1;
+    }
+    ph.add_reductions(red);
+    return used;
+}
+
+void finish_iteration(ParaJob &p, const double *sums_host)
+{
+    const double *h = sums_host + p.sums_off;
+    const int k = p.k;
+    double num = 0.0, den = 0.0;
+    for (int s = 1; s <= p.ns; s++) {
+        if (s >= k + 1) {
+            num += h[2 * s];
+            p.slot_u2[s] = h[2 * s + 1];
+        }
+        den += p.slot_u2[s];
+    }
+    const double inc = den > 0 ? std::sqrt(num / den) : std::sqrt(num);
+    p.R.increment[k] = inc;
+    p.R.iterations = k + 1;
+    p.k = k + 1;
+    if (!std::isfinite(inc)) {
+        p.running = false;
+        p.status = CCFV_ENONFINITE;
+        return;
+    }
+    if (p.tol > 0 && inc <= p.tol) {
+        p.R.converged = 1;
+        p.running = false;
+    }
+    if (p.k >= p.ns) {  // every slice exact
+        p.R.converged = 1;
+        p.running = false;
+    }
+    if (p.k >= p.kmax) p.running = false;
+}
+
+}  // namespace rt
+}  // namespace ccfv

@@ -431,13 +431,13 @@ void clear_graphs(ccfv_ctx *c)
 void free_para(ccfv_ctx *c)
 {
     clear_graphs(c);
-    hipFree(c->U);
-    hipFree(c->G);
-    hipFree(c->F);
-    hipFree(c->Wk);
-    hipFree(c->partial);
-    hipFree(c->slot_sums);
-    if (c->h_sums) hipHostFree(c->h_sums);
+    (void)hipFree(c->U);
+    (void)hipFree(c->G);
+    (void)hipFree(c->F);
+    (void)hipFree(c->Wk);
+    (void)hipFree(c->partial);
+    (void)hipFree(c->slot_sums);
+    if (c->h_sums) (void)hipHostFree(c->h_sums);
     c->U = c->G = c->F = c->Wk = c->partial = c->slot_sums = c->h_sums = nullptr;
     c->para_ns = 0;
 }
@@ -573,8 +573,8 @@ int ccfv_create(const ccfv_problem *p, ccfv_ctx **out)
     if (const char *s = getenv("CCFV_PIPE_GROUP")) c->pipe_group = std::max(1, atoi(s));
     if (const char *s = getenv("CCFV_GRAPHS")) c->graphs = atoi(s) != 0;
     if (const char *s = getenv("CCFV_PLAIN_GRAPHS")) c->plain_graphs = atoi(s) != 0;
-    hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream);
-    hipMemsetAsync(c->state, 0, sizeof(double) * c->ncell, c->stream);
+    (void)hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream);
+    (void)hipMemsetAsync(c->state, 0, sizeof(double) * c->ncell, c->stream);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) {
         ccfv_destroy(c);
         return hip_fail(e, "context init");
@@ -596,11 +596,11 @@ void ccfv_destroy(ccfv_ctx *c)
         (void)hipFree(kv.second.dev);
         (void)hipFree(kv.second.line_dev);
     }
-    hipFree(c->state);
-    hipFree(c->d_flag);
-    if (c->h_flag) hipHostFree(c->h_flag);
-    if (c->ev0) hipEventDestroy(c->ev0);
-    if (c->ev1) hipEventDestroy(c->ev1);
+    (void)hipFree(c->state);
+    (void)hipFree(c->d_flag);
+    if (c->h_flag) (void)hipHostFree(c->h_flag);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -669,7 +669,7 @@ int ccfv_exact_to_host(ccfv_ctx *c, const ccfv_ic *ic, double t, double *host)
         e = hipMemcpyAsync(host, tmp, sizeof(double) * c->ncell,
                            hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(tmp);
+    (void)hipFree(tmp);
     if (e != hipSuccess) return hip_fail(e, "exact_to_host");
     return CCFV_OK;
 }

@@ -112,3 +112,62 @@ def test_batch_without_device_fails_loudly():
     with pytest.raises(ccfv.CcfvError) as ei:
         ccfv.Batch([T.rotation_problem(64)])
     assert ei.value.code == T.CCFV_EDEVICE
+
+
+def test_ctypes_mirrors_match_the_c_header(tmp_path):
+    """sizeof / offsetof of every ABI struct, from the C compiler, against
+    the ctypes mirrors."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    structs = {"ccfv_problem": T.Problem, "ccfv_ic": T.IC,
+               "ccfv_parareal_report": T.ParaReport,
+               "ccfv_kernel_stat": T.KernelStat,
+               "ccfv_server_config": T.ServerConfig, "ccfv_request": T.Request,
+               "ccfv_server_status": T.ServerStatus}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ccfv.h"',
+             "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "sizes.c"
+    src.write_text("\n".join(lines))
+    exe = str(tmp_path / "sizes")
+    subprocess.run([gcc, "-std=c99", "-I", os.path.join(ROOT, "include"), str(src),
+                    "-o", exe], check=True)
+    got = dict(l.split() for l in subprocess.run([exe], capture_output=True,
+                                                 text=True).stdout.splitlines())
+    for cname, cls in structs.items():
+        assert int(got[cname]) == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(cls, f).offset, (cname, f)
+
+
+def test_server_arguments_checked_before_the_device():
+    lib = ccfv.load()
+    assert lib.ccfv_server_create(None, None) == T.CCFV_EINVAL
+    cfg = T.ServerConfig(1 << 20, 5000, 0, 0, 0)
+    h = ctypes.c_void_p()
+    assert lib.ccfv_server_create(ctypes.byref(cfg), ctypes.byref(h)) == T.CCFV_EINVAL
+    assert b"max_active" in lib.ccfv_last_error()
+    assert lib.ccfv_server_step(None, None) == T.CCFV_EINVAL
+    lib.ccfv_server_destroy(None)
+
+
+def test_request_bytes_is_the_resident_footprint():
+    from ccfv import workloads
+    s = workloads.rotation_solve(256)
+    r = s.request(kmax=4, tol=0.0)
+    b = ccfv.request_bytes(r)
+    f = 8 * 256 * 256
+    # U (ns+1) + G ns + F ns + work field, plus the increment partials
+    assert b == (3 * 16 + 2) * f + 2 * 8 * 128 * 17
+    bad = s.request(kmax=4, tol=0.0)
+    bad.problem.nx = 36
+    with pytest.raises(ccfv.CcfvError) as ei:
+        ccfv.request_bytes(bad)
+    assert ei.value.code == T.CCFV_EINVAL

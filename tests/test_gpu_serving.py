@@ -1,0 +1,125 @@
+"""GPU: the serving engine.  Each request's result is the same alone and in
+a mix of requests of other sizes at other iterations (bitwise), and the
+same as ccfv_parareal_solve on its own context; admission keeps the
+resident state within the memory budget."""
+import numpy as np
+import pytest
+
+import ccfv
+from ccfv import serving, workloads
+from ccfv import types as T
+
+pytestmark = pytest.mark.gpu
+
+
+def small_mix():
+    rng = np.random.default_rng(11)
+    out = [workloads.ragged_solve(rng, grids=(64, 128), slices=(4, 8), t_end=0.25)
+           for _ in range(6)]
+    out.append(workloads.Solve(T.rotation_problem(64), T.rotation_ic(), 0.0, 0.25,
+                               32, 8, kind="rotation"))
+    out.append(workloads.Solve(T.reference_problem(32, 0.01), T.reference_ic(), 0.0,
+                               0.3, 24, 6, kind="reference"))
+    out.append(workloads.Solve(T.swirl_problem(48, T=1.0), T.cosine_bell_ic(), 0.0,
+                               0.5, 32, 4, kind="swirl"))
+    return out
+
+
+def drain(srv):
+    out = {}
+    for _ in range(10000):
+        st = srv.step()
+        while True:
+            rid = srv.next_done()
+            if rid < 0:
+                break
+            out[rid] = srv.result(rid)
+        if st.active == 0 and st.queued == 0:
+            return out
+    raise AssertionError("engine did not drain")
+
+
+def alone(s, kmax, tol):
+    with ccfv.Server(mem_budget=1 << 30, max_active=4) as srv:
+        rid = srv.submit(s.request(kmax, tol))
+        return drain(srv)[rid]
+
+
+@pytest.mark.parametrize("kmax,tol", [(3, 0.0), (8, 1e-9)])
+def test_same_alone_and_in_the_mix(gpu, kmax, tol):
+    mix = small_mix()
+    ref = [alone(s, kmax, tol) for s in mix]
+    got = {}
+    with ccfv.Server(mem_budget=1 << 30, max_active=64) as srv:
+        ids = {}
+        # staggered: requests join while earlier ones are mid-iteration
+        for i, s in enumerate(mix):
+            ids[srv.submit(s.request(kmax, tol))] = i
+            if i % 3 == 2:
+                srv.step()
+                while (rid := srv.next_done()) >= 0:
+                    got[ids[rid]] = srv.result(rid)
+        for rid, r in drain(srv).items():
+            got[ids[rid]] = r
+    for i, s in enumerate(mix):
+        u, rep, st = got[i]
+        ua, ra, sa = ref[i]
+        assert st == sa, i
+        assert np.array_equal(u, ua), (i, s.kind)
+        assert rep.iterations == ra.iterations and rep.converged == ra.converged
+        assert list(rep.increments) == list(ra.increments), i
+        # and the single-problem entry point
+        with gpu.Context(s.problem) as c:
+            c.fill_exact(s.ic, s.t0)
+            rs = c.parareal_solve(s.t0, s.t1, s.nt_fine, s.n_slices, kmax, tol)
+            assert np.array_equal(c.get_state(), u), (i, s.kind)
+            assert rs.iterations == rep.iterations
+            np.testing.assert_allclose(rs.increments, rep.increments, rtol=1e-12)
+
+
+def test_admission_by_memory_budget(gpu):
+    s = workloads.rotation_solve(128, t_end=0.125, slices=8)
+    r = s.request(4, 0.0)
+    b = ccfv.request_bytes(r)
+    budget = 2 * b + 4096
+    ref = alone(s, 4, 0.0)
+    with ccfv.Server(mem_budget=budget, max_active=16) as srv:
+        ids = [srv.submit(r) for _ in range(5)]
+        st = srv.step()
+        assert st.active == 2 and st.queued == 3
+        assert st.mem_used <= st.mem_budget <= budget
+        res = drain(srv)
+    assert sorted(res) == sorted(ids)
+    for rid in ids:
+        assert np.array_equal(res[rid][0], ref[0])
+
+
+def test_explicit_initial_state_equals_ic(gpu):
+    s = small_mix()[0]
+    with gpu.Context(s.problem) as c:
+        u0 = c.exact(s.ic, s.t0)
+    with ccfv.Server(mem_budget=1 << 28, max_active=4) as srv:
+        a = srv.submit(s.request(2, 0.0))
+        b = srv.submit(s.request(2, 0.0), u0)
+        res = drain(srv)
+    assert np.array_equal(res[a][0], res[b][0])
+
+
+def test_request_larger_than_budget_is_refused(gpu):
+    s = workloads.rotation_solve(512)
+    with ccfv.Server(mem_budget=1 << 24, max_active=4) as srv:
+        with pytest.raises(ccfv.CcfvError) as ei:
+            srv.submit(s.request(2, 0.0))
+        assert ei.value.code == T.CCFV_EBUSY
+
+
+def test_open_loop_run_on_the_engine(gpu):
+    mix = small_mix()
+    arr = [(0.002 * i, s.request(3, 0.0), None) for i, s in enumerate(mix)]
+    with ccfv.Server(mem_budget=1 << 30, max_active=8) as srv:
+        res = serving.run(srv, arr)
+    assert len(res.records) == len(mix)
+    assert all(r.status == T.CCFV_OK for r in res.records)
+    assert all(r.latency > 0 for r in res.records)
+    s = res.summary()
+    assert s["p50_ms"] <= s["p99_ms"] <= s["p999_ms"]
