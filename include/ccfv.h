@@ -208,11 +208,17 @@ int ccfv_parareal_solve(ccfv_ctx *ctx, double t0, double t1, int32_t nt_fine,
  *                       same times, slices and shapes (0, default: issued
  *                       directly, which keeps the overlap across phases).
  *   CCFV_OPT_PLAIN_GRAPHS 1: ccfv_plain_solve captures the whole solve as a
- *                       hipGraph and replays it for identical solves. */
+ *                       hipGraph and replays it for identical solves.
+ *   CCFV_OPT_FUSE       1 (default): in a run of steps the last x sweep of a
+ *                       step and the first x sweep of the next run as one
+ *                       launch on the same lines (one HBM round trip less);
+ *                       0: one launch per sweep.  Both give identical
+ *                       results. */
 #define CCFV_OPT_PIPELINE     1
 #define CCFV_OPT_PIPE_GROUP   2
 #define CCFV_OPT_GRAPHS       3
 #define CCFV_OPT_PLAIN_GRAPHS 4
+#define CCFV_OPT_FUSE         5
 int ccfv_set_option(ccfv_ctx *ctx, int32_t key, int64_t value);
 int ccfv_get_option(ccfv_ctx *ctx, int32_t key, int64_t *value);
 

@@ -46,6 +46,7 @@ OPT_PIPELINE = 1
 OPT_PIPE_GROUP = 2
 OPT_GRAPHS = 3
 OPT_PLAIN_GRAPHS = 4
+OPT_FUSE = 5
 
 
 class Problem(ctypes.Structure):

@@ -3,6 +3,7 @@
 // the batch (ccfv_batch.hip) and the serving engine (ccfv_server.hip).
 #pragma once
 
+#include <functional>
 #include <map>
 #include <tuple>
 #include <vector>
@@ -18,6 +19,7 @@ struct StageItem {
     int bc;
     bool post;
     Epi epi;
+    bool fuse;  // x: also the next step's first x sub-step (out2)
     Tile tile;
     RaggedItem item;
 };
@@ -29,7 +31,7 @@ struct Launch {
     Dir dir = DIR_X;
     Tile tile{0, 0};
     int ch = 1, bc = 0;
-    bool fast = false, post = false;
+    bool fast = false, post = false, fuse = false;
     Epi epi = EPI_NONE;
     int nitems = 0, nblocks = 0;
     size_t lds = 0;
@@ -64,24 +66,20 @@ struct PhaseRunner {
     int run(Phase &ph);
 };
 
-// the three stages of one Strang step of one problem
-struct StepSpec {
+// A run of chained Strang steps of one problem (as run_steps), emitted into
+// a phase round by round: round r holds the x launch that ends step r-1 and
+// starts step r (one fused launch) and the y sweep of step r.
+struct StepRun {
     ccfv_ctx *c;
-    const SolveTable *tx, *ty;
-    const double *in;
-    double *out;
-    int nf;
-    double t0, tDT, jdt, h;
-    int field0;
-    Epi epi;
-    double *g_cache;
-    const double *f_in;
-    double *u_next;
-    double *partial;
+    int count;
     int shape;  // 1: batch launch shapes (fine propagators), 0: one-field
+    std::function<StepDesc(int)> desc;
 };
-void add_step_items(const StepSpec &s, std::vector<StageItem> st[3]);
-void flush_round(Phase &ph, std::vector<StageItem> st[3]);
+// emits rounds [r0, r1) of every run (round `count` is a run's last x
+// sweep); returns CCFV_OK or the table lookup's error
+int emit_runs(Phase &ph, const std::vector<StepRun> &runs, int r0, int r1);
+// rounds a set of runs takes
+int run_rounds(const std::vector<StepRun> &runs);
 
 // launch shapes every grid of a batch can use (multiples of 8 cells)
 void shared_tiles(ccfv_ctx *c);
@@ -111,8 +109,9 @@ int job_setup(ParaJob &j, ccfv_ctx *c, double t0, double t1, int nt_fine, int ns
 // Appends, for every job, its k = 0 coarse sweep if not yet issued and then
 // (if it has iterations left) one iteration.  Jobs advance in lock-step
 // rounds; slot sums go to sums_dev at each job's sums_off (2 (ns+1) doubles),
-// assigned here.  Returns the number of doubles of sums_dev used.
-size_t append_iteration(Phase &ph, std::vector<ParaJob *> &jobs, double *sums_dev);
+// assigned here; *used: the doubles of sums_dev used.
+int append_iteration(Phase &ph, std::vector<ParaJob *> &jobs, double *sums_dev,
+                     size_t *used);
 // reads an iteration's slot sums (host copy of sums_dev) into the job
 void finish_iteration(ParaJob &j, const double *sums_host);
 

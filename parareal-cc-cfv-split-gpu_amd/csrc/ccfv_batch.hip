@@ -113,32 +113,30 @@ int ccfv_batch_plain(ccfv_batch *b, const double *t0, const double *t1,
     int rc;
     if ((rc = join_contexts(b))) return rc;
     std::vector<double> dt(np);
-    std::vector<const SolveTable *> tx(np), ty(np);
-    int ntmax = 0;
     for (int p = 0; p < np; p++) {
         if (nt[p] < 1 || !(t1[p] > t0[p])) return fail(CCFV_EINVAL, "bad time interval");
         dt[p] = (t1[p] - t0[p]) / nt[p];
-        if ((rc = get_table(b->ctx[p], DIR_X, dt[p], &tx[p])) ||
-            (rc = get_table(b->ctx[p], DIR_Y, dt[p], &ty[p])))
+        // tables up front: host work stays out of the launch sequence
+        const SolveTable *t;
+        if ((rc = get_table(b->ctx[p], DIR_X, dt[p], &t)) ||
+            (rc = get_table(b->ctx[p], DIR_Y, dt[p], &t)))
             return rc;
-        ntmax = std::max(ntmax, (int)nt[p]);
+    }
+    std::vector<StepRun> runs;
+    for (int p = 0; p < np; p++) {
+        ccfv_ctx *c = b->ctx[p];
+        const double a = t0[p], h = dt[p];
+        runs.push_back(StepRun{c, (int)nt[p], 0, [c, a, h](int n) {
+                                   return StepDesc{c->state, c->state, 1, a + n * h, 0.0, 0,
+                                                   0.0, h};
+                               }});
     }
     // phases of 64 rounds keep the descriptor upload small
     Phase ph;
-    std::vector<StageItem> st[3];
-    for (int r0 = 0; r0 < ntmax; r0 += 64) {
+    const int rounds = run_rounds(runs);
+    for (int r0 = 0; r0 < rounds; r0 += 64) {
         ph.clear();
-        for (int r = r0; r < std::min(ntmax, r0 + 64); r++) {
-            for (int p = 0; p < np; p++) {
-                if (r >= nt[p]) continue;
-                ccfv_ctx *c = b->ctx[p];
-                StepSpec s{c, tx[p], ty[p], c->state, c->state, 1, t0[p] + r * dt[p],
-                           0.0, 0.0, dt[p], 0, EPI_NONE, nullptr, nullptr, nullptr,
-                           nullptr, 0};
-                add_step_items(s, st);
-            }
-            flush_round(ph, st);
-        }
+        if ((rc = emit_runs(ph, runs, r0, std::min(rounds, r0 + 64)))) return rc;
         if ((rc = b->run.run(ph))) return rc;
     }
     HIPCHK(hipStreamSynchronize(b->run.stream));
@@ -180,7 +178,8 @@ int ccfv_batch_parareal(ccfv_batch *b, const double *t0, const double *t1,
     std::vector<ParaJob *> live;
     for (ParaJob &j : jobs) live.push_back(&j);
     while (!live.empty()) {
-        const size_t used = append_iteration(ph, live, b->sums_dev);
+        size_t used = 0;
+        if ((rc = append_iteration(ph, live, b->sums_dev, &used))) return rc;
         if ((rc = b->run.run(ph))) return rc;
         ph.clear();
         if (used) {

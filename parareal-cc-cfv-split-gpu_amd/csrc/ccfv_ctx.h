@@ -3,6 +3,7 @@
 // layer (ccfv_batch.hip).
 #pragma once
 
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -36,6 +37,8 @@ struct ccfv_ctx {
     // launch shapes change
     bool graphs = false;
     bool plain_graphs = false;
+    // one launch for the last x sweep of a step and the first of the next
+    bool fuse = true;
     int tiles_version = 0;
     std::map<std::string, hipGraphExec_t> gcache;
     std::vector<hipEvent_t> sync_ev;  // cross-stream events (no timing)
@@ -69,6 +72,28 @@ int get_table(ccfv_ctx *c, Dir dir, double h, const SolveTable **out);
 SweepArgs make_args(ccfv_ctx *c, Dir dir, int substep, const SolveTable &T,
                     double h, double t0, double tDT, int field0, double jdt);
 bool post_flag(const ccfv_ctx *c);
+// One Strang step of nf fields (in -> out) with an optional Parareal
+// epilogue on its last x sweep.
+struct StepDesc {
+    const double *in;
+    double *out;
+    int nf;
+    double t0, tDT;
+    int field0;
+    double jdt, h;
+    Epi epi = EPI_NONE;
+    double *g_cache = nullptr;
+    const double *f_in = nullptr;
+    double *u_next = nullptr;
+    double *partial = nullptr;
+};
+// `count` steps in order; where step i+1 reads step i's result (out, or
+// u_next with an epilogue) the last x sweep of i and the first of i+1 are
+// one launch.  pre(i) runs before the launch that starts step i, post(i)
+// after the launch that ends it.
+int run_steps(ccfv_ctx *c, int count, const std::function<StepDesc(int)> &desc,
+              const std::function<int(int)> &pre = {},
+              const std::function<int(int)> &post = {});
 int step_fields(ccfv_ctx *c, const double *in, double *out, int nf, double t0,
                 double tDT, int field0, double jdt, double h, Epi epi = EPI_NONE,
                 double *g_cache = nullptr, const double *f_in = nullptr,

@@ -66,6 +66,12 @@ struct SweepArgs {
     const double *f_in;   // F(U_n^k)
     double *u_next;       // U[n+1] (read old, write new)
     double *partial;      // 2 doubles per block: sum (du)^2, sum u^2
+    // fused x launch: the last x sub-step of one Strang step followed, on
+    // the same lines in LDS, by the first x sub-step of the next step (its
+    // field offset and time offset), whose result goes to out2
+    double *out2;
+    int field0_2;
+    double jdt2;
 };
 
 struct Tile {
@@ -88,18 +94,19 @@ struct RaggedItem {
 // general sweeps (ccfv_sweep_generic.hip)
 hipError_t launch_sweep_generic(Dir dir, int bc, bool post, Epi epi,
                                 const SweepArgs &a, int nfields, Tile tile,
-                                hipStream_t s);
+                                hipStream_t s, bool fuse = false);
 size_t sweep_lds_bytes_generic(int n, Tile tile);
 // uniform-displacement sweeps (ccfv_sweep_fast.hip); periodic only
 hipError_t launch_sweep_fast(Dir dir, Epi epi, const SweepArgs &a, int nfields,
-                             Tile tile, hipStream_t s);
+                             Tile tile, hipStream_t s, bool fuse = false);
 size_t sweep_lds_bytes_fast(int n, Tile tile);
 // ragged launches over many problems of one line class (same chunk CH,
-// boundary mode and path); items and prefix are device arrays
-hipError_t launch_ragged_fast(Dir dir, Epi epi, int ch, const RaggedItem *items,
-                              const int *prefix, int nitems, int nblocks,
-                              Tile tile, size_t lds, hipStream_t s);
-hipError_t launch_ragged_generic(Dir dir, int bc, bool post, Epi epi, int ch,
+// boundary mode and path); items and prefix are device arrays.  fuse: x
+// launches that also run the next step's first x sub-step (SweepArgs.out2)
+hipError_t launch_ragged_fast(Dir dir, Epi epi, int ch, bool fuse,
+                              const RaggedItem *items, const int *prefix, int nitems,
+                              int nblocks, Tile tile, size_t lds, hipStream_t s);
+hipError_t launch_ragged_generic(Dir dir, int bc, bool post, Epi epi, int ch, bool fuse,
                                  const RaggedItem *items, const int *prefix,
                                  int nitems, int nblocks, Tile tile, size_t lds,
                                  hipStream_t s);

@@ -53,6 +53,19 @@ __device__ __forceinline__ Times sub_times(const SweepArgs &a, int f)
     return t;
 }
 
+// sampling times of the first x sub-step of the next step in a fused x
+// launch: the arithmetic of sub_times with field0_2 and jdt2
+__device__ __forceinline__ Times sub_times_next(const SweepArgs &a, int f)
+{
+#pragma clang fp contract(off)
+    double tn = a.t0 + (double)(a.field0_2 + f) * a.tDT;
+    double tl = tn + a.jdt2;
+    Times t;
+    t.lo = tl;
+    t.hi = tl + 0.5 * a.h;
+    return t;
+}
+
 // foot of face q of the line with corner coordinates o0, o1
 template <int DIR>
 __device__ __forceinline__ double face_foot(const SweepArgs &a, const Times &t,
@@ -462,16 +475,34 @@ __device__ __forceinline__ int xcd_tile(int bx, int gx)
     return (bx & 7) * (gx >> 3) + (bx >> 3);
 }
 
+// Loads of a tile are issued TILE_UNROLL at a time per work-item before any
+// of them is written to LDS, so each wave keeps several requests in flight.
+constexpr int TILE_UNROLL = 8;
+
 __device__ __forceinline__ void load_tile_x(const double *in, int ld, int n,
                                             int j0, int W, double *lds, int LP)
 {
     const int hw = W >> 1, sh = log2i(hw);
-    for (int idx = threadIdx.x; idx < hw * n; idx += blockDim.x) {
-        int c2 = idx & (hw - 1), i = idx >> sh;
-        double2 v = *reinterpret_cast<const double2 *>(in + (size_t)i * ld + j0 +
-                                                       2 * c2);
-        lds[(2 * c2) * LP + pidx(i)] = v.x;
-        lds[(2 * c2 + 1) * LP + pidx(i)] = v.y;
+    const int total = hw * n, step = blockDim.x;
+    for (int b0 = threadIdx.x; b0 < total; b0 += TILE_UNROLL * step) {
+        double2 v[TILE_UNROLL];
+#pragma unroll
+        for (int u = 0; u < TILE_UNROLL; u++) {
+            const int idx = b0 + u * step;
+            if (idx < total) {
+                const int c2 = idx & (hw - 1), i = idx >> sh;
+                v[u] = *reinterpret_cast<const double2 *>(in + (size_t)i * ld + j0 + 2 * c2);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < TILE_UNROLL; u++) {
+            const int idx = b0 + u * step;
+            if (idx < total) {
+                const int c2 = idx & (hw - 1), i = idx >> sh;
+                lds[(2 * c2) * LP + pidx(i)] = v[u].x;
+                lds[(2 * c2 + 1) * LP + pidx(i)] = v[u].y;
+            }
+        }
     }
 }
 
@@ -492,34 +523,59 @@ __device__ __forceinline__ void store_tile_x(double *out, int ld, int n, int j0,
 //   EPI_INIT   : G cache <- g, U[n+1] <- g
 //   EPI_CORRECT: U[n+1] <- g + (F - G_old) (T2S4_Parareal.cu:779),
 //                G cache <- g, block partials of |dU|^2 and |U|^2
-template <int EPI>
+// WB (fused launches): the corrected U[n+1] also replaces the tile in LDS,
+// where the next step's first x sub-step reads it.
+template <int EPI, bool WB = false>
 __device__ __forceinline__ void epilogue_x(const SweepArgs &a, int n, int j0,
-                                           int W, const double *lds, int LP,
+                                           int W, double *lds, int LP,
                                            int slot)
 {
     const int hw = W >> 1, sh = log2i(hw);
+    const int total = hw * n, step = blockDim.x;
+    constexpr int U = 4;
     double s0 = 0.0, s1 = 0.0;
-    for (int idx = threadIdx.x; idx < hw * n; idx += blockDim.x) {
-        int c2 = idx & (hw - 1), i = idx >> sh;
-        size_t gi = (size_t)i * a.ld + j0 + 2 * c2;
-        double2 g;
-        g.x = lds[(2 * c2) * LP + pidx(i)];
-        g.y = lds[(2 * c2 + 1) * LP + pidx(i)];
-        if (EPI == EPI_INIT) {
-            *reinterpret_cast<double2 *>(a.g_cache + gi) = g;
-            *reinterpret_cast<double2 *>(a.u_next + gi) = g;
-        } else {
-            double2 go = *reinterpret_cast<const double2 *>(a.g_cache + gi);
-            double2 fv = *reinterpret_cast<const double2 *>(a.f_in + gi);
-            double2 uo = *reinterpret_cast<const double2 *>(a.u_next + gi);
-            double2 un;
-            un.x = g.x + (fv.x - go.x);
-            un.y = g.y + (fv.y - go.y);
-            *reinterpret_cast<double2 *>(a.g_cache + gi) = g;
-            *reinterpret_cast<double2 *>(a.u_next + gi) = un;
-            double d0 = un.x - uo.x, d1 = un.y - uo.y;
-            s0 += d0 * d0 + d1 * d1;
-            s1 += un.x * un.x + un.y * un.y;
+    for (int b0 = threadIdx.x; b0 < total; b0 += U * step) {
+        double2 go[U], fv[U], uo[U];
+        if (EPI == EPI_CORRECT) {
+            // the three inputs of U elements in flight together
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int idx = b0 + u * step;
+                if (idx < total) {
+                    const int c2 = idx & (hw - 1), i = idx >> sh;
+                    const size_t gi = (size_t)i * a.ld + j0 + 2 * c2;
+                    go[u] = *reinterpret_cast<const double2 *>(a.g_cache + gi);
+                    fv[u] = *reinterpret_cast<const double2 *>(a.f_in + gi);
+                    uo[u] = *reinterpret_cast<const double2 *>(a.u_next + gi);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int idx = b0 + u * step;
+            if (idx >= total) continue;
+            const int c2 = idx & (hw - 1), i = idx >> sh;
+            const size_t gi = (size_t)i * a.ld + j0 + 2 * c2;
+            double2 g;
+            g.x = lds[(2 * c2) * LP + pidx(i)];
+            g.y = lds[(2 * c2 + 1) * LP + pidx(i)];
+            if (EPI == EPI_INIT) {
+                *reinterpret_cast<double2 *>(a.g_cache + gi) = g;
+                *reinterpret_cast<double2 *>(a.u_next + gi) = g;
+            } else {
+                double2 un;
+                un.x = g.x + (fv[u].x - go[u].x);
+                un.y = g.y + (fv[u].y - go[u].y);
+                *reinterpret_cast<double2 *>(a.g_cache + gi) = g;
+                *reinterpret_cast<double2 *>(a.u_next + gi) = un;
+                if (WB) {
+                    lds[(2 * c2) * LP + pidx(i)] = un.x;
+                    lds[(2 * c2 + 1) * LP + pidx(i)] = un.y;
+                }
+                double d0 = un.x - uo[u].x, d1 = un.y - uo[u].y;
+                s0 += d0 * d0 + d1 * d1;
+                s1 += un.x * un.x + un.y * un.y;
+            }
         }
     }
     if (EPI == EPI_CORRECT) block_reduce2(s0, s1, a.partial, slot);
@@ -544,12 +600,28 @@ __device__ __forceinline__ void load_tile_y(const double *in, int ld, int n,
                                             int W, double *lds, int LP)
 {
     const int hn = n >> 1;
-    for (int idx = threadIdx.x; idx < W * hn; idx += blockDim.x) {
-        int r, j2;
-        split_row(idx, hn, r, j2);
-        double2 v = *reinterpret_cast<const double2 *>(in + (size_t)r * ld + 2 * j2);
-        lds[r * LP + pidx(2 * j2)] = v.x;
-        lds[r * LP + pidx(2 * j2 + 1)] = v.y;
+    const int total = W * hn, step = blockDim.x;
+    for (int b0 = threadIdx.x; b0 < total; b0 += TILE_UNROLL * step) {
+        double2 v[TILE_UNROLL];
+#pragma unroll
+        for (int u = 0; u < TILE_UNROLL; u++) {
+            const int idx = b0 + u * step;
+            if (idx < total) {
+                int r, j2;
+                split_row(idx, hn, r, j2);
+                v[u] = *reinterpret_cast<const double2 *>(in + (size_t)r * ld + 2 * j2);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < TILE_UNROLL; u++) {
+            const int idx = b0 + u * step;
+            if (idx < total) {
+                int r, j2;
+                split_row(idx, hn, r, j2);
+                lds[r * LP + pidx(2 * j2)] = v[u].x;
+                lds[r * LP + pidx(2 * j2 + 1)] = v[u].y;
+            }
+        }
     }
 }
 

@@ -54,17 +54,17 @@ void Phase::add_sweep_stage(std::vector<StageItem> &items)
     // one launch shares the kernel instance (line chunk, boundary mode, path,
     // step-3 variant, epilogue) and the block shape
     struct Key {
-        int ch, bc, fast, post, epi, w, nw;
+        int ch, bc, fast, post, epi, fuse, w, nw;
         bool operator<(const Key &o) const
         {
-            return std::tie(ch, bc, fast, post, epi, w, nw) <
-                   std::tie(o.ch, o.bc, o.fast, o.post, o.epi, o.w, o.nw);
+            return std::tie(ch, bc, fast, post, epi, fuse, w, nw) <
+                   std::tie(o.ch, o.bc, o.fast, o.post, o.epi, o.fuse, o.w, o.nw);
         }
     };
     std::map<Key, std::vector<const StageItem *>> groups;
     for (const StageItem &s : items) {
         Key k{chunk_for(s.item.a.n), s.bc, s.item.a.lineS ? 1 : 0, s.post ? 1 : 0,
-              (int)s.epi, s.tile.w, s.tile.nw};
+              (int)s.epi, s.fuse ? 1 : 0, s.tile.w, s.tile.nw};
         groups[k].push_back(&s);
     }
     for (auto &kv : groups) {
@@ -87,6 +87,7 @@ void Phase::add_sweep_stage(std::vector<StageItem> &items)
         L.bc = k.bc;
         L.fast = k.fast != 0;
         L.post = k.post != 0;
+        L.fuse = k.fuse != 0;
         L.epi = (Epi)k.epi;
         L.nitems = (int)it.size();
         L.nblocks = prefix.back();
@@ -196,10 +197,11 @@ int PhaseRunner::run(Phase &ph)
         switch (L.kind) {
         case Launch::SWEEP: {
             const RaggedItem *it = (const RaggedItem *)(d + L.items);
-            e = L.fast ? launch_ragged_fast(L.dir, L.epi, L.ch, it, pf, L.nitems, L.nblocks,
-                                            L.tile, L.lds, stream)
-                       : launch_ragged_generic(L.dir, L.bc, L.post, L.epi, L.ch, it, pf,
-                                               L.nitems, L.nblocks, L.tile, L.lds, stream);
+            e = L.fast ? launch_ragged_fast(L.dir, L.epi, L.ch, L.fuse, it, pf, L.nitems,
+                                            L.nblocks, L.tile, L.lds, stream)
+                       : launch_ragged_generic(L.dir, L.bc, L.post, L.epi, L.ch, L.fuse, it,
+                                               pf, L.nitems, L.nblocks, L.tile, L.lds,
+                                               stream);
             break;
         }
         case Launch::CORR:
@@ -222,66 +224,99 @@ int PhaseRunner::run(Phase &ph)
 }
 
 // ---------------------------------------------------------------------------
-// Strang steps as stage items
+// runs of Strang steps as stage items
 // ---------------------------------------------------------------------------
-void add_step_items(const StepSpec &s, std::vector<StageItem> st[3])
+namespace {
+
+StageItem stage_item(ccfv_ctx *c, Dir dir, int substep, const SolveTable &T,
+                     const StepDesc &d, int shape)
 {
-    ccfv_ctx *c = s.c;
-    // fine propagators use the batch shapes; the coarse chain and the plain
-    // sweep the one-field shapes, which also fix the epilogue's partial sums
-    const int bt = s.shape;
-    const long long fs = (long long)c->ncell;
-    StageItem a;
-    a.dir = DIR_X;
-    a.bc = c->P.bc;
-    a.post = false;
-    a.epi = EPI_NONE;
-    a.tile = c->tile[DIR_X][bt];
-    a.item.a = make_args(c, DIR_X, 1, *s.tx, s.h, s.t0, s.tDT, s.field0, s.jdt);
-    a.item.a.in = s.in;
-    a.item.a.out = s.out;
-    a.item.a.fstride = fs;
-    a.item.nfields = s.nf;
-    a.item.ntiles = c->P.ny / a.tile.w;
-    st[0].push_back(a);
-    StageItem b;
-    b.dir = DIR_Y;
-    b.bc = c->P.bc;
-    b.post = false;
-    b.epi = EPI_NONE;
-    b.tile = c->tile[DIR_Y][bt];
-    b.item.a = make_args(c, DIR_Y, 2, *s.ty, s.h, s.t0, s.tDT, s.field0, s.jdt);
-    b.item.a.in = s.out;
-    b.item.a.out = s.out;
-    b.item.a.fstride = fs;
-    b.item.nfields = s.nf;
-    b.item.ntiles = c->P.nx / b.tile.w;
-    st[1].push_back(b);
-    StageItem x;
-    x.dir = DIR_X;
-    x.bc = c->P.bc;
-    x.post = post_flag(c);
-    x.epi = s.epi;
-    x.tile = c->tile[DIR_X][bt];
-    x.item.a = make_args(c, DIR_X, 3, *s.tx, s.h, s.t0, s.tDT, s.field0, s.jdt);
-    x.item.a.in = s.out;
-    x.item.a.out = s.out;
-    x.item.a.fstride = fs;
-    x.item.a.g_cache = s.g_cache;
-    x.item.a.f_in = s.f_in;
-    x.item.a.u_next = s.u_next;
-    x.item.a.partial = s.partial;
-    x.item.nfields = s.nf;
-    x.item.ntiles = c->P.ny / x.tile.w;
-    st[2].push_back(x);
+    StageItem s;
+    s.dir = dir;
+    s.bc = c->P.bc;
+    s.post = dir == DIR_X && substep == 3 && post_flag(c);
+    s.epi = substep == 3 ? d.epi : EPI_NONE;
+    s.fuse = false;
+    s.tile = c->tile[dir][shape];
+    SweepArgs &a = s.item.a;
+    a = make_args(c, dir, substep, T, d.h, d.t0, d.tDT, d.field0, d.jdt);
+    a.in = substep == 1 ? d.in : d.out;
+    a.out = d.out;
+    a.fstride = (long long)c->ncell;
+    if (substep == 3) {
+        a.g_cache = d.g_cache;
+        a.f_in = d.f_in;
+        a.u_next = d.u_next;
+        a.partial = d.partial;
+    }
+    s.item.nfields = d.nf;
+    s.item.ntiles = (dir == DIR_X ? c->P.ny : c->P.nx) / s.tile.w;
+    return s;
 }
 
-void flush_round(Phase &ph, std::vector<StageItem> st[3])
+bool fusable(const StepDesc &d, const StepDesc &n)
 {
-    for (int i = 0; i < 3; i++) {
-        if (!st[i].empty()) ph.add_sweep_stage(st[i]);
-        st[i].clear();
+    const double *res = d.epi == EPI_NONE ? d.out : d.u_next;
+    return n.in == res && n.h == d.h && n.nf == d.nf && n.tDT == d.tDT && n.t0 == d.t0;
+}
+
+}  // namespace
+
+int run_rounds(const std::vector<StepRun> &runs)
+{
+    int r = 0;
+    for (const StepRun &u : runs)
+        if (u.count > 0) r = std::max(r, u.count + 1);
+    return r;
+}
+
+int emit_runs(Phase &ph, const std::vector<StepRun> &runs, int r0, int r1)
+{
+    // stage 0: x launches (x1 of step 0, fused x3(r-1)+x1(r), last x3);
+    // stage 1: x1 of steps that cannot be fused with their predecessor;
+    // stage 2: y sweeps
+    std::vector<StageItem> st[3];
+    int rc;
+    for (int r = r0; r < r1; r++) {
+        for (const StepRun &u : runs) {
+            if (r > u.count || u.count <= 0) continue;
+            ccfv_ctx *c = u.c;
+            const SolveTable *tx, *ty;
+            if (r == 0) {
+                StepDesc d = u.desc(0);
+                if ((rc = get_table(c, DIR_X, d.h, &tx))) return rc;
+                st[0].push_back(stage_item(c, DIR_X, 1, *tx, d, u.shape));
+            } else {
+                StepDesc d = u.desc(r - 1);
+                if ((rc = get_table(c, DIR_X, d.h, &tx))) return rc;
+                StageItem x3 = stage_item(c, DIR_X, 3, *tx, d, u.shape);
+                if (r < u.count) {
+                    StepDesc n = u.desc(r);
+                    if (fusable(d, n)) {
+                        x3.fuse = true;
+                        x3.item.a.out2 = n.out;
+                        x3.item.a.field0_2 = n.field0;
+                        x3.item.a.jdt2 = n.jdt;
+                    } else {
+                        const SolveTable *tn;
+                        if ((rc = get_table(c, DIR_X, n.h, &tn))) return rc;
+                        st[1].push_back(stage_item(c, DIR_X, 1, *tn, n, u.shape));
+                    }
+                }
+                st[0].push_back(x3);
+            }
+            if (r < u.count) {
+                StepDesc d = u.desc(r);
+                if ((rc = get_table(c, DIR_Y, d.h, &ty))) return rc;
+                st[2].push_back(stage_item(c, DIR_Y, 2, *ty, d, u.shape));
+            }
+        }
+        for (int i = 0; i < 3; i++) {
+            if (!st[i].empty()) ph.add_sweep_stage(st[i]);
+            st[i].clear();
+        }
     }
+    return CCFV_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -322,30 +357,46 @@ int job_setup(ParaJob &j, ccfv_ctx *c, double t0, double t1, int nt_fine, int ns
 
 namespace {
 
-StepSpec coarse_spec(const ParaJob &p, int n, Epi epi)
+// the steps of a job (the schedule of ccfv_parareal_solve)
+StepDesc job_init_step(const ParaJob &p, int n)
 {
     const size_t N = p.c->ncell;
-    StepSpec s{p.c, p.cx, p.cy, p.c->U + n * N, p.c->Wk, 1, p.t0, p.DT, 0.0, p.DT, n, epi,
-               p.c->G + n * N, epi == EPI_CORRECT ? p.c->F + n * N : nullptr,
-               p.c->U + (n + 1) * N,
-               epi == EPI_CORRECT ? p.c->partial + 2 * (size_t)p.nb * (n + 1) : nullptr, 0};
-    return s;
+    return StepDesc{p.c->U + n * N, p.c->Wk, 1, p.t0, p.DT, n, 0.0, p.DT, EPI_INIT,
+                    p.c->G + n * N, nullptr, p.c->U + (n + 1) * N, nullptr};
+}
+
+StepDesc job_fine_step(const ParaJob &p, int j)
+{
+    const size_t N = p.c->ncell;
+    const double *src = j == 0 ? p.c->U + p.k * N : p.c->F + p.k * N;
+    return StepDesc{src, p.c->F + p.k * N, p.ns - p.k, p.t0, p.DT, p.k, j * p.dt, p.dt};
+}
+
+StepDesc job_chain_step(const ParaJob &p, int n)
+{
+    const size_t N = p.c->ncell;
+    return StepDesc{p.c->U + n * N, p.c->Wk, 1, p.t0, p.DT, n, 0.0, p.DT, EPI_CORRECT,
+                    p.c->G + n * N, p.c->F + n * N, p.c->U + (n + 1) * N,
+                    p.c->partial + 2 * (size_t)p.nb * (n + 1)};
+}
+
+int emit_all(Phase &ph, const std::vector<StepRun> &runs)
+{
+    return emit_runs(ph, runs, 0, run_rounds(runs));
 }
 
 }  // namespace
 
-size_t append_iteration(Phase &ph, std::vector<ParaJob *> &jobs, double *sums_dev)
+int append_iteration(Phase &ph, std::vector<ParaJob *> &jobs, double *sums_dev,
+                     size_t *used_out)
 {
-    std::vector<StageItem> st[3];
+    int rc;
     // k = 0 coarse sweep: U[n+1] = G(U[n]), G cached (T2S4_Parareal.cu:1264-1280)
-    int rounds = 0;
+    std::vector<StepRun> runs;
     for (ParaJob *p : jobs)
-        if (!p->started) rounds = std::max(rounds, p->ns);
-    for (int n = 0; n < rounds; n++) {
-        for (ParaJob *p : jobs)
-            if (!p->started && n < p->ns) add_step_items(coarse_spec(*p, n, EPI_INIT), st);
-        flush_round(ph, st);
-    }
+        if (!p->started)
+            runs.push_back(StepRun{p->c, p->ns, 0, [p](int n) { return job_init_step(*p, n); }});
+    if ((rc = emit_all(ph, runs))) return rc;
     for (ParaJob *p : jobs)
         if (!p->started) {
             p->started = true;
@@ -357,20 +408,10 @@ size_t append_iteration(Phase &ph, std::vector<ParaJob *> &jobs, double *sums_de
         if (p->running && p->k < p->kmax) it.push_back(p);
     // fine propagators of slices k..ns-1, all slices of a job as one batch
     // of fields (T2S4_Parareal.cu:1319-1324)
-    int Jmax = 0;
-    for (ParaJob *p : it) Jmax = std::max(Jmax, p->J);
-    for (int jj = 0; jj < Jmax; jj++) {
-        for (ParaJob *p : it) {
-            if (jj >= p->J) continue;
-            const size_t N = p->c->ncell;
-            const double *src = jj == 0 ? p->c->U + p->k * N : p->c->F + p->k * N;
-            StepSpec s{p->c, p->fx, p->fy, src, p->c->F + p->k * N, p->ns - p->k, p->t0,
-                       p->DT, jj * p->dt, p->dt, p->k, EPI_NONE, nullptr, nullptr, nullptr,
-                       nullptr, 1};
-            add_step_items(s, st);
-        }
-        flush_round(ph, st);
-    }
+    runs.clear();
+    for (ParaJob *p : it)
+        runs.push_back(StepRun{p->c, p->J, 1, [p](int j) { return job_fine_step(*p, j); }});
+    if ((rc = emit_all(ph, runs))) return rc;
     // slot k+1 by the correction alone (T2S4_Parareal.cu:1327-1330)
     std::vector<RaggedCorr> corr;
     std::vector<int> cblocks;
@@ -384,16 +425,12 @@ size_t append_iteration(Phase &ph, std::vector<ParaJob *> &jobs, double *sums_de
     }
     ph.add_corrections(corr, cblocks);
     // coarse chain from slice k+1 with the correction fused into its last
-    // sweep; round r is step k+1+r of every job
-    int crounds = 0;
-    for (ParaJob *p : it) crounds = std::max(crounds, p->ns - p->k - 1);
-    for (int r = 0; r < crounds; r++) {
-        for (ParaJob *p : it) {
-            const int n = p->k + 1 + r;
-            if (n < p->ns) add_step_items(coarse_spec(*p, n, EPI_CORRECT), st);
-        }
-        flush_round(ph, st);
-    }
+    // sweep
+    runs.clear();
+    for (ParaJob *p : it)
+        runs.push_back(StepRun{p->c, p->ns - p->k - 1, 0,
+                               [p](int i) { return job_chain_step(*p, p->k + 1 + i); }});
+    if ((rc = emit_all(ph, runs))) return rc;
     // per-slot increment sums of slots k+1..ns
     std::vector<RaggedReduce> red;
     size_t used = 0;
@@ -406,7 +443,8 @@ size_t append_iteration(Phase &ph, std::vector<ParaJob *> &jobs, double *sums_de
         p->R.coarse_steps_run += p->ns - p->k - 1;
     }
     ph.add_reductions(red);
-    return used;
+    *used_out = used;
+    return CCFV_OK;
 }
 
 void finish_iteration(ParaJob &p, const double *sums_host)

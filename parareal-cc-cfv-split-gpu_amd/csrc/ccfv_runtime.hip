@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 #include <utility>
@@ -335,50 +336,144 @@ bool post_flag(const ccfv_ctx *c)
     return (c->P.flags & CCFV_FLAG_STEP3_POSTREMAP) != 0;
 }
 
-// One Strang step x(h/2) -> y(h) -> x(h/2) on nf fields (GpuOneStep,
-// T2S4_Parareal.cu:785-968).  The first sweep reads `in` and writes `out`;
-// the other two work in place on `out` (every sweep is line-local).
+hipError_t run_sweep_fused(ccfv_ctx *c, bool post, Epi epi, const SweepArgs &a, int nf,
+                           Tile t)
+{
+    if (nf <= 0) return hipSuccess;
+    if (a.lineS) return launch_sweep_fast(DIR_X, epi, a, nf, t, c->cur, true);
+    return launch_sweep_generic(DIR_X, c->P.bc, post, epi, a, nf, t, c->cur, true);
+}
+
+namespace {
+
+// profiling classes and algorithmic bytes of the x launches of a run
+int x_kind(Epi epi, bool batch)
+{
+    return epi != EPI_NONE ? CCFV_KSTAT_X_EPILOG
+                           : (batch ? CCFV_KSTAT_X_BATCH : CCFV_KSTAT_X_SINGLE);
+}
+
+double x_bytes(const ccfv_ctx *c, Epi epi, int nf, bool fused)
+{
+    const double f = 8.0 * (double)c->ncell;
+    // epilogues: read G_old, F, U_old; write G, U[n+1] (EPI_INIT: write G,
+    // U[n+1]); a fused launch also writes the next step's first x sweep
+    double b = epi == EPI_CORRECT ? 6.0 * f : epi == EPI_INIT ? 3.0 * f : 2.0 * f * nf;
+    if (fused && epi != EPI_NONE) b += f;
+    return b;
+}
+
+}  // namespace
+
+hipError_t timed_sweep_fused(ccfv_ctx *c, int kind, double bytes, bool post, Epi epi,
+                             const SweepArgs &a, int nf, Tile t)
+{
+    if (!c->prof) return run_sweep_fused(c, post, epi, a, nf, t);
+    if (c->recs.size() > 4096) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream2);
+        if (e != hipSuccess) return e;
+        drain_records(c);
+    }
+    ccfv_ctx::Rec r{kind, bytes, pool_event(c), pool_event(c)};
+    if (r.a) (void)hipEventRecord(r.a, c->cur);
+    hipError_t e = run_sweep_fused(c, post, epi, a, nf, t);
+    if (r.b) (void)hipEventRecord(r.b, c->cur);
+    c->recs.push_back(r);
+    return e;
+}
+
+// Strang steps x(h/2) -> y(h) -> x(h/2) on nf fields (GpuOneStep,
+// T2S4_Parareal.cu:785-968).  A step's first sweep reads `in` and writes
+// `out`; the other two work in place on `out` (every sweep is line-local).
+// In a run of steps where step i+1 reads step i's result, the last x sweep
+// of step i and the first x sweep of step i+1 are one launch over the same
+// lines: the same arithmetic on the same values, one HBM round trip and one
+// launch fewer per step.
+int run_steps(ccfv_ctx *c, int count, const std::function<StepDesc(int)> &desc,
+              const std::function<int(int)> &pre, const std::function<int(int)> &post)
+{
+    if (count <= 0) return CCFV_OK;
+    const bool postflag = post_flag(c);
+    const long long fs = (long long)c->ncell;
+    int rc;
+    StepDesc D = desc(0);
+    const SolveTable *tx, *ty;
+    if ((rc = get_table(c, DIR_X, D.h, &tx)) || (rc = get_table(c, DIR_Y, D.h, &ty)))
+        return rc;
+    auto x1_args = [&](const StepDesc &S, const SolveTable &T) {
+        SweepArgs a = make_args(c, DIR_X, 1, T, S.h, S.t0, S.tDT, S.field0, S.jdt);
+        a.in = S.in;
+        a.out = S.out;
+        a.fstride = fs;
+        return a;
+    };
+    if (pre && (rc = pre(0))) return rc;
+    {
+        const bool batch = D.nf > 1;
+        HIPCHK(timed_sweep(c, batch ? CCFV_KSTAT_X_BATCH : CCFV_KSTAT_X_SINGLE,
+                           x_bytes(c, EPI_NONE, D.nf, false), DIR_X, false, EPI_NONE,
+                           x1_args(D, *tx), D.nf, c->tile[DIR_X][batch]));
+    }
+    for (int i = 0; i < count; i++) {
+        const bool batch = D.nf > 1;
+        SweepArgs b = make_args(c, DIR_Y, 2, *ty, D.h, D.t0, D.tDT, D.field0, D.jdt);
+        b.in = D.out;
+        b.out = D.out;
+        b.fstride = fs;
+        HIPCHK(timed_sweep(c, batch ? CCFV_KSTAT_Y_BATCH : CCFV_KSTAT_Y_SINGLE,
+                           16.0 * (double)c->ncell * D.nf, DIR_Y, false, EPI_NONE, b, D.nf,
+                           c->tile[DIR_Y][batch]));
+        SweepArgs x3 = make_args(c, DIR_X, 3, *tx, D.h, D.t0, D.tDT, D.field0, D.jdt);
+        x3.in = D.out;
+        x3.out = D.out;
+        x3.fstride = fs;
+        x3.g_cache = D.g_cache;
+        x3.f_in = D.f_in;
+        x3.u_next = D.u_next;
+        x3.partial = D.partial;
+        const Tile xt = c->tile[DIR_X][batch];
+        if (i + 1 == count) {
+            HIPCHK(timed_sweep(c, x_kind(D.epi, batch), x_bytes(c, D.epi, D.nf, false),
+                               DIR_X, postflag, D.epi, x3, D.nf, xt));
+            if (post && (rc = post(i))) return rc;
+            break;
+        }
+        StepDesc N = desc(i + 1);
+        const double *res = D.epi == EPI_NONE ? D.out : D.u_next;
+        const bool fuse = c->fuse && N.in == res && N.h == D.h && N.nf == D.nf &&
+                          N.tDT == D.tDT && N.t0 == D.t0;
+        if (pre && (rc = pre(i + 1))) return rc;
+        if (fuse) {
+            x3.out2 = N.out;
+            x3.field0_2 = N.field0;
+            x3.jdt2 = N.jdt;
+            HIPCHK(timed_sweep_fused(c, x_kind(D.epi, batch), x_bytes(c, D.epi, D.nf, true),
+                                     postflag, D.epi, x3, D.nf, xt));
+            if (post && (rc = post(i))) return rc;
+        } else {
+            HIPCHK(timed_sweep(c, x_kind(D.epi, batch), x_bytes(c, D.epi, D.nf, false),
+                               DIR_X, postflag, D.epi, x3, D.nf, xt));
+            if (post && (rc = post(i))) return rc;
+            if (N.h != D.h &&
+                ((rc = get_table(c, DIR_X, N.h, &tx)) || (rc = get_table(c, DIR_Y, N.h, &ty))))
+                return rc;
+            const bool nb = N.nf > 1;
+            HIPCHK(timed_sweep(c, nb ? CCFV_KSTAT_X_BATCH : CCFV_KSTAT_X_SINGLE,
+                               x_bytes(c, EPI_NONE, N.nf, false), DIR_X, false, EPI_NONE,
+                               x1_args(N, *tx), N.nf, c->tile[DIR_X][nb]));
+        }
+        D = N;
+    }
+    return CCFV_OK;
+}
+
 int step_fields(ccfv_ctx *c, const double *in, double *out, int nf, double t0,
                 double tDT, int field0, double jdt, double h, Epi epi,
                 double *g_cache, const double *f_in, double *u_next, double *partial)
 {
-    const SolveTable *tx, *ty;
-    int rc = get_table(c, DIR_X, h, &tx);
-    if (rc) return rc;
-    rc = get_table(c, DIR_Y, h, &ty);
-    if (rc) return rc;
-    const bool batch = nf > 1;
-    const bool post = post_flag(c);
-    const long long fs = (long long)c->ncell;
-    SweepArgs a = make_args(c, DIR_X, 1, *tx, h, t0, tDT, field0, jdt);
-    a.in = in;
-    a.out = out;
-    a.fstride = fs;
-    const double fbytes = 16.0 * (double)c->ncell * nf;
-    HIPCHK(timed_sweep(c, batch ? CCFV_KSTAT_X_BATCH : CCFV_KSTAT_X_SINGLE, fbytes,
-                       DIR_X, false, EPI_NONE, a, nf, c->tile[DIR_X][batch]));
-    SweepArgs b = make_args(c, DIR_Y, 2, *ty, h, t0, tDT, field0, jdt);
-    b.in = out;
-    b.out = out;
-    b.fstride = fs;
-    HIPCHK(timed_sweep(c, batch ? CCFV_KSTAT_Y_BATCH : CCFV_KSTAT_Y_SINGLE, fbytes,
-                       DIR_Y, false, EPI_NONE, b, nf, c->tile[DIR_Y][batch]));
-    SweepArgs x3 = make_args(c, DIR_X, 3, *tx, h, t0, tDT, field0, jdt);
-    x3.in = out;
-    x3.out = out;
-    x3.fstride = fs;
-    x3.g_cache = g_cache;
-    x3.f_in = f_in;
-    x3.u_next = u_next;
-    x3.partial = partial;
-    int kind = epi != EPI_NONE ? CCFV_KSTAT_X_EPILOG
-                               : (batch ? CCFV_KSTAT_X_BATCH : CCFV_KSTAT_X_SINGLE);
-    double xbytes = epi == EPI_CORRECT ? 48.0 * (double)c->ncell
-                    : epi == EPI_INIT  ? 24.0 * (double)c->ncell
-                                       : fbytes;
-    HIPCHK(timed_sweep(c, kind, xbytes, DIR_X, post, epi, x3, nf,
-                       c->tile[DIR_X][batch]));
-    return CCFV_OK;
+    StepDesc d{in, out, nf, t0, tDT, field0, jdt, h, epi, g_cache, f_in, u_next, partial};
+    return run_steps(c, 1, [&](int) { return d; });
 }
 
 int check_flag(ccfv_ctx *c)
@@ -573,6 +668,7 @@ int ccfv_create(const ccfv_problem *p, ccfv_ctx **out)
     if (const char *s = getenv("CCFV_PIPE_GROUP")) c->pipe_group = std::max(1, atoi(s));
     if (const char *s = getenv("CCFV_GRAPHS")) c->graphs = atoi(s) != 0;
     if (const char *s = getenv("CCFV_PLAIN_GRAPHS")) c->plain_graphs = atoi(s) != 0;
+    if (const char *s = getenv("CCFV_FUSE")) c->fuse = atoi(s) != 0;
     (void)hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream);
     (void)hipMemsetAsync(c->state, 0, sizeof(double) * c->ncell, c->stream);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) {
@@ -698,10 +794,10 @@ int ccfv_fine(ccfv_ctx *c, double tn, double DT, int32_t J)
     if (!(DT > 0.0) || !std::isfinite(DT))
         return fail(CCFV_EINVAL, "DT must be finite and > 0");
     double dt = DT / (double)J;
-    for (int j = 0; j < J; j++) {
-        int rc = step_fields(c, c->state, c->state, 1, tn, 0.0, 0, j * dt, dt);
-        if (rc) return rc;
-    }
+    int rc = run_steps(c, J, [&](int j) {
+        return StepDesc{c->state, c->state, 1, tn, 0.0, 0, j * dt, dt};
+    });
+    if (rc) return rc;
     return check_flag(c);
 }
 
@@ -715,11 +811,9 @@ int ccfv_plain_solve(ccfv_ctx *c, double t0, double t1, int32_t nt)
     double dt = (t1 - t0) / nt;
     c->cur = c->stream;
     auto body = [&]() -> int {
-        for (int n = 0; n < nt; n++) {
-            int rc = step_fields(c, c->state, c->state, 1, t0 + n * dt, 0.0, 0, 0.0, dt);
-            if (rc) return rc;
-        }
-        return CCFV_OK;
+        return run_steps(c, nt, [&](int n) {
+            return StepDesc{c->state, c->state, 1, t0 + n * dt, 0.0, 0, 0.0, dt};
+        });
     };
     if (c->plain_graphs && !c->prof) {
         // the whole solve as one hipGraph, replayed by later identical solves
@@ -940,6 +1034,33 @@ struct ParaPlan {
     double tol;
 };
 
+// the steps of the Parareal schedule
+StepDesc init_step(ccfv_ctx *c, const ParaPlan &pl, int n)
+{
+    // k = 0 coarse sweep: U[n+1] = G(U[n]), G cached
+    const size_t N = c->ncell;
+    return StepDesc{c->U + n * N, c->Wk, 1, pl.t0, pl.DT, n, 0.0, pl.DT, EPI_INIT,
+                    c->G + n * N, nullptr, c->U + (n + 1) * N, nullptr};
+}
+
+StepDesc fine_step(ccfv_ctx *c, const ParaPlan &pl, int lo, int nf, int j)
+{
+    // step j of the fine propagators of slices lo..lo+nf-1, one batch
+    const size_t N = c->ncell;
+    const double *src = j == 0 ? c->U + lo * N : c->F + lo * N;
+    return StepDesc{src, c->F + lo * N, nf, pl.t0, pl.DT, lo, j * pl.dt, pl.dt};
+}
+
+StepDesc chain_step(ccfv_ctx *c, const ParaPlan &pl, int n)
+{
+    // coarse step of slice n with the correction of slot n+1 fused
+    const size_t N = c->ncell;
+    const int nb = c->chain_blocks;
+    return StepDesc{c->U + n * N, c->Wk, 1, pl.t0, pl.DT, n, 0.0, pl.DT, EPI_CORRECT,
+                    c->G + n * N, c->F + n * N, c->U + (n + 1) * N,
+                    c->partial + 2 * (size_t)nb * (n + 1)};
+}
+
 // Parareal with everything on one stream: k=0 coarse sweep, then per
 // iteration the batched fine propagators of the slices that can still change
 // and the coarse chain with the correction fused into its last sweep.
@@ -950,30 +1071,19 @@ int parareal_sequential(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R
     const int ns = pl.ns, J = pl.J, nb = c->chain_blocks;
     int rc;
     c->cur = c->stream;
-    for (int n = 0; n < ns; n++) {
-        rc = step_fields(c, c->U + n * N, c->Wk, 1, pl.t0, pl.DT, n, 0.0, pl.DT,
-                         EPI_INIT, c->G + n * N, nullptr, c->U + (n + 1) * N, nullptr);
-        if (rc) return rc;
-    }
+    if ((rc = run_steps(c, ns, [&](int n) { return init_step(c, pl, n); }))) return rc;
     R.coarse_steps_run = ns;
     int k;
     for (k = 0; k < pl.kmax; k++) {
         const int nf = ns - k;
-        for (int j = 0; j < J; j++) {
-            const double *src = j == 0 ? c->U + k * N : c->F + k * N;
-            rc = step_fields(c, src, c->F + k * N, nf, pl.t0, pl.DT, k, j * pl.dt, pl.dt);
-            if (rc) return rc;
-        }
+        if ((rc = run_steps(c, J, [&](int j) { return fine_step(c, pl, k, nf, j); })))
+            return rc;
         R.fine_steps_run += (double)nf * J;
         HIPCHK(launch_correction(c->G + k * N, c->G + k * N, c->F + k * N,
                                  c->U + (k + 1) * N, N,
                                  c->partial + 2 * (size_t)nb * (k + 1), nb, c->stream));
-        for (int n = k + 1; n < ns; n++) {
-            rc = step_fields(c, c->U + n * N, c->Wk, 1, pl.t0, pl.DT, n, 0.0, pl.DT,
-                             EPI_CORRECT, c->G + n * N, c->F + n * N,
-                             c->U + (n + 1) * N, c->partial + 2 * (size_t)nb * (n + 1));
-            if (rc) return rc;
-        }
+        rc = run_steps(c, ns - k - 1, [&](int i) { return chain_step(c, pl, k + 1 + i); });
+        if (rc) return rc;
         R.coarse_steps_run += ns - k - 1;
         double *sums = c->slot_sums + (size_t)k * 2 * (ns + 1);
         HIPCHK(launch_reduce_segments(c->partial, nb, k + 1, ns - k, sums, c->stream));
@@ -1049,13 +1159,14 @@ int parareal_pipelined(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R,
             const int lo = std::max(g0(g), k);
             if (lo >= g1(g)) continue;
             HIPCHK(hipStreamWaitEvent(sF, ev(par, 0, g), 0));
-            for (int j = 0; j < J; j++) {
-                const double *src = j == 0 ? c->U + lo * N : c->F + lo * N;
-                int r2 = step_fields(c, src, c->F + lo * N, g1(g) - lo, pl.t0, pl.DT,
-                                     lo, j * pl.dt, pl.dt);
-                if (r2) return r2;
-                if (j == 0 && !graphs) HIPCHK(hipEventRecord(ev(par, 1, g), sF));
-            }
+            int r2 = run_steps(
+                c, J, [&](int j) { return fine_step(c, pl, lo, g1(g) - lo, j); }, {},
+                [&](int j) -> int {
+                    // U of the group has been read once step 0 is done
+                    if (j == 0 && !graphs) HIPCHK(hipEventRecord(ev(par, 1, g), sF));
+                    return CCFV_OK;
+                });
+            if (r2) return r2;
             if (!graphs) HIPCHK(hipEventRecord(ev(par, 2, g), sF));
         }
         c->cur = sC;
@@ -1072,14 +1183,12 @@ int parareal_pipelined(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R,
     // k = 0 coarse sweep; group g is ready for fine(0) after step g1-1
     auto enqueue_init = [&]() -> int {
         c->cur = sC;
-        for (int n = 0; n < ns; n++) {
-            int r2 = step_fields(c, c->U + n * N, c->Wk, 1, pl.t0, pl.DT, n, 0.0, pl.DT,
-                                 EPI_INIT, c->G + n * N, nullptr, c->U + (n + 1) * N,
-                                 nullptr);
-            if (r2) return r2;
-            if (n == g1(grp(n)) - 1) HIPCHK(hipEventRecord(ev(0, 0, grp(n)), sC));
-        }
-        return CCFV_OK;
+        return run_steps(
+            c, ns, [&](int n) { return init_step(c, pl, n); }, {},
+            [&](int n) -> int {
+                if (n == g1(grp(n)) - 1) HIPCHK(hipEventRecord(ev(0, 0, grp(n)), sC));
+                return CCFV_OK;
+            });
     };
     auto enqueue_chain = [&](int k) -> int {
         const int par = k & 1, npar = (k + 1) & 1;
@@ -1094,19 +1203,24 @@ int parareal_pipelined(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R,
         HIPCHK(launch_correction(c->G + k * N, c->G + k * N, c->F + k * N,
                                  c->U + (k + 1) * N, N,
                                  c->partial + 2 * (size_t)nb * (k + 1), nb, sC));
-        for (int n = k + 1; n < ns; n++) {
-            const int g = grp(n);
-            if (!graphs && (n == k + 1 || n == g0(g))) {
-                HIPCHK(hipStreamWaitEvent(sC, ev(par, 2, g), 0));
-                if (g + 1 < G) HIPCHK(hipStreamWaitEvent(sC, ev(par, 1, g + 1), 0));
-            }
-            int r2 = step_fields(c, c->U + n * N, c->Wk, 1, pl.t0, pl.DT, n, 0.0,
-                                 pl.DT, EPI_CORRECT, c->G + n * N, c->F + n * N,
-                                 c->U + (n + 1) * N,
-                                 c->partial + 2 * (size_t)nb * (n + 1));
-            if (r2) return r2;
-            if (n == g1(g) - 1) HIPCHK(hipEventRecord(ev(npar, 0, g), sC));
-        }
+        // waits come before the launch that starts step n (which also ends
+        // step n-1), records after the launch that ends step n
+        int r2 = run_steps(
+            c, ns - k - 1, [&](int i) { return chain_step(c, pl, k + 1 + i); },
+            [&](int i) -> int {
+                const int n = k + 1 + i, g = grp(n);
+                if (!graphs && (n == k + 1 || n == g0(g))) {
+                    HIPCHK(hipStreamWaitEvent(sC, ev(par, 2, g), 0));
+                    if (g + 1 < G) HIPCHK(hipStreamWaitEvent(sC, ev(par, 1, g + 1), 0));
+                }
+                return CCFV_OK;
+            },
+            [&](int i) -> int {
+                const int n = k + 1 + i, g = grp(n);
+                if (n == g1(g) - 1) HIPCHK(hipEventRecord(ev(npar, 0, g), sC));
+                return CCFV_OK;
+            });
+        if (r2) return r2;
         double *sums = c->slot_sums + (size_t)k * 2 * (ns + 1);
         HIPCHK(launch_reduce_segments(c->partial, nb, k + 1, ns - k, sums, sC));
         HIPCHK(hipMemcpyAsync(c->h_sums + (size_t)k * 2 * (ns + 1), sums,
@@ -1294,6 +1408,10 @@ int ccfv_set_option(ccfv_ctx *c, int32_t key, int64_t value)
         c->plain_graphs = value != 0;
         clear_graphs(c);
         return CCFV_OK;
+    case CCFV_OPT_FUSE:
+        c->fuse = value != 0;
+        clear_graphs(c);
+        return CCFV_OK;
     }
     return fail(CCFV_EINVAL, "unknown option");
 }
@@ -1306,6 +1424,7 @@ int ccfv_get_option(ccfv_ctx *c, int32_t key, int64_t *value)
     case CCFV_OPT_PIPE_GROUP: *value = c->pipe_group; return CCFV_OK;
     case CCFV_OPT_GRAPHS: *value = c->graphs ? 1 : 0; return CCFV_OK;
     case CCFV_OPT_PLAIN_GRAPHS: *value = c->plain_graphs ? 1 : 0; return CCFV_OK;
+    case CCFV_OPT_FUSE: *value = c->fuse ? 1 : 0; return CCFV_OK;
     }
     return fail(CCFV_EINVAL, "unknown option");
 }

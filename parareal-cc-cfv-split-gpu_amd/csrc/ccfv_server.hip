@@ -285,7 +285,8 @@ int ccfv_server_step(ccfv_server *s, ccfv_server_status *st)
         std::vector<ParaJob *> jobs;
         for (Request *q : s->active) jobs.push_back(&q->job);
         Phase ph;
-        const size_t used = append_iteration(ph, jobs, s->sums_dev);
+        size_t used = 0;
+        if ((rc = append_iteration(ph, jobs, s->sums_dev, &used))) return rc;
         if ((rc = s->run.run(ph))) return rc;
         if (used)
             HIPCHK(hipMemcpyAsync(s->sums_host, s->sums_dev, sizeof(double) * used,

@@ -173,7 +173,10 @@ __device__ __forceinline__ void fast_line(const SweepArgs &a, double *cs,
     }
 }
 
-template <int CH, int EPI>
+// FUSE: after the sweep (and its epilogue, whose result stays in LDS) the
+// same lines take the next step's first x sub-step, with the same table
+// (one step size) and the same per-line shift, and go to out2.
+template <int CH, int EPI, bool FUSE>
 __device__ __forceinline__ void fast_x_body(const SweepArgs &a, int field, int tile,
                                            int W, int NW, double *lds)
 {
@@ -192,8 +195,19 @@ __device__ __forceinline__ void fast_x_body(const SweepArgs &a, int field, int t
     for (int l = wave; l < W; l += NW)
         fast_line<CH, REG>(a, lds + l * LP, j0 + l, lane, tb, mfl, mbl, pneg0);
     __syncthreads();
-    if (EPI == EPI_NONE) store_tile_x(a.out + foff, a.ld, n, j0, W, lds, LP);
-    else epilogue_x<EPI>(a, n, j0, W, lds, LP, tile);
+    if (!FUSE) {
+        if (EPI == EPI_NONE) store_tile_x(a.out + foff, a.ld, n, j0, W, lds, LP);
+        else epilogue_x<EPI>(a, n, j0, W, lds, LP, tile);
+        return;
+    }
+    if (EPI != EPI_NONE) {
+        epilogue_x<EPI, true>(a, n, j0, W, lds, LP, tile);
+        __syncthreads();
+    }
+    for (int l = wave; l < W; l += NW)
+        fast_line<CH, REG>(a, lds + l * LP, j0 + l, lane, tb, mfl, mbl, pneg0);
+    __syncthreads();
+    store_tile_x(a.out2 + foff, a.ld, n, j0, W, lds, LP);
 }
 
 template <int CH>
@@ -218,11 +232,12 @@ __device__ __forceinline__ void fast_y_body(const SweepArgs &a, int field, int t
     store_tile_y(a.out + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
 }
 
-template <int CH, int EPI>
+template <int CH, int EPI, bool FUSE>
 __global__ __launch_bounds__(256) void fast_x_kernel(SweepArgs a, int W, int NW)
 {
     extern __shared__ double lds[];
-    fast_x_body<CH, EPI>(a, blockIdx.y, xcd_tile(blockIdx.x, gridDim.x), W, NW, lds);
+    fast_x_body<CH, EPI, FUSE>(a, blockIdx.y, xcd_tile(blockIdx.x, gridDim.x), W, NW,
+                               lds);
 }
 
 template <int CH>
@@ -233,7 +248,7 @@ __global__ __launch_bounds__(256) void fast_y_kernel(SweepArgs a, int W, int NW)
 }
 
 // ragged: one launch over many problems of the same line class
-template <int CH, int EPI>
+template <int CH, int EPI, bool FUSE>
 __global__ __launch_bounds__(256) void ragged_fast_x(const RaggedItem *items,
                                                       const int *prefix, int nitems,
                                                       int W, int NW)
@@ -245,7 +260,7 @@ __global__ __launch_bounds__(256) void ragged_fast_x(const RaggedItem *items,
     const int nt = items[i].ntiles;
     const int field = local / nt;
     const SweepArgs a = items[i].a;
-    fast_x_body<CH, EPI>(a, field, local - field * nt, W, NW, lds);
+    fast_x_body<CH, EPI, FUSE>(a, field, local - field * nt, W, NW, lds);
 }
 
 template <int CH>
@@ -283,28 +298,38 @@ static hipError_t launch(K k, dim3 grid, dim3 block, size_t lds, hipStream_t s,
     return hipGetLastError();
 }
 
+template <int CH, bool FUSE>
+static hipError_t dispatch_x(Epi epi, dim3 grid, dim3 block, size_t lds, hipStream_t s,
+                             const SweepArgs &a, Tile t)
+{
+    if (epi == EPI_NONE)
+        return launch(fast_x_kernel<CH, EPI_NONE, FUSE>, grid, block, lds, s, a, t);
+    if (epi == EPI_INIT)
+        return launch(fast_x_kernel<CH, EPI_INIT, FUSE>, grid, block, lds, s, a, t);
+    return launch(fast_x_kernel<CH, EPI_CORRECT, FUSE>, grid, block, lds, s, a, t);
+}
+
 template <int CH>
-static hipError_t dispatch(Dir dir, Epi epi, const SweepArgs &a, int nf, Tile t,
-                           hipStream_t s)
+static hipError_t dispatch(Dir dir, Epi epi, bool fuse, const SweepArgs &a, int nf,
+                           Tile t, hipStream_t s)
 {
     dim3 grid(a.nl / t.w, nf), block(64 * t.nw);
     size_t lds = sweep_lds_bytes_fast(a.n, t);
     if (dir == DIR_Y) return launch(fast_y_kernel<CH>, grid, block, lds, s, a, t);
-    if (epi == EPI_NONE) return launch(fast_x_kernel<CH, EPI_NONE>, grid, block, lds, s, a, t);
-    if (epi == EPI_INIT) return launch(fast_x_kernel<CH, EPI_INIT>, grid, block, lds, s, a, t);
-    return launch(fast_x_kernel<CH, EPI_CORRECT>, grid, block, lds, s, a, t);
+    return fuse ? dispatch_x<CH, true>(epi, grid, block, lds, s, a, t)
+                : dispatch_x<CH, false>(epi, grid, block, lds, s, a, t);
 }
 
 hipError_t launch_sweep_fast(Dir dir, Epi epi, const SweepArgs &a, int nfields,
-                             Tile tile, hipStream_t s)
+                             Tile tile, hipStream_t s, bool fuse)
 {
     switch (chunk_for(a.n)) {
-    case 1: return dispatch<1>(dir, epi, a, nfields, tile, s);
-    case 2: return dispatch<2>(dir, epi, a, nfields, tile, s);
-    case 4: return dispatch<4>(dir, epi, a, nfields, tile, s);
-    case 8: return dispatch<8>(dir, epi, a, nfields, tile, s);
-    case 16: return dispatch<16>(dir, epi, a, nfields, tile, s);
-    case 32: return dispatch<32>(dir, epi, a, nfields, tile, s);
+    case 1: return dispatch<1>(dir, epi, fuse, a, nfields, tile, s);
+    case 2: return dispatch<2>(dir, epi, fuse, a, nfields, tile, s);
+    case 4: return dispatch<4>(dir, epi, fuse, a, nfields, tile, s);
+    case 8: return dispatch<8>(dir, epi, fuse, a, nfields, tile, s);
+    case 16: return dispatch<16>(dir, epi, fuse, a, nfields, tile, s);
+    case 32: return dispatch<32>(dir, epi, fuse, a, nfields, tile, s);
     }
     return hipErrorInvalidValue;
 }
@@ -325,35 +350,43 @@ static hipError_t launch_r(K k, int nblocks, Tile t, size_t lds, hipStream_t s,
     return hipGetLastError();
 }
 
+template <int CH, bool FUSE>
+static hipError_t dispatch_rx(Epi epi, const RaggedItem *items, const int *prefix,
+                              int nitems, int nblocks, Tile t, size_t lds, hipStream_t s)
+{
+    if (epi == EPI_NONE)
+        return launch_r(ragged_fast_x<CH, EPI_NONE, FUSE>, nblocks, t, lds, s, items,
+                        prefix, nitems);
+    if (epi == EPI_INIT)
+        return launch_r(ragged_fast_x<CH, EPI_INIT, FUSE>, nblocks, t, lds, s, items,
+                        prefix, nitems);
+    return launch_r(ragged_fast_x<CH, EPI_CORRECT, FUSE>, nblocks, t, lds, s, items,
+                    prefix, nitems);
+}
+
 template <int CH>
-static hipError_t dispatch_r(Dir dir, Epi epi, const RaggedItem *items,
+static hipError_t dispatch_r(Dir dir, Epi epi, bool fuse, const RaggedItem *items,
                              const int *prefix, int nitems, int nblocks, Tile t,
                              size_t lds, hipStream_t s)
 {
     if (dir == DIR_Y)
         return launch_r(ragged_fast_y<CH>, nblocks, t, lds, s, items, prefix, nitems);
-    if (epi == EPI_NONE)
-        return launch_r(ragged_fast_x<CH, EPI_NONE>, nblocks, t, lds, s, items, prefix,
-                        nitems);
-    if (epi == EPI_INIT)
-        return launch_r(ragged_fast_x<CH, EPI_INIT>, nblocks, t, lds, s, items, prefix,
-                        nitems);
-    return launch_r(ragged_fast_x<CH, EPI_CORRECT>, nblocks, t, lds, s, items, prefix,
-                    nitems);
+    return fuse ? dispatch_rx<CH, true>(epi, items, prefix, nitems, nblocks, t, lds, s)
+                : dispatch_rx<CH, false>(epi, items, prefix, nitems, nblocks, t, lds, s);
 }
 
-hipError_t launch_ragged_fast(Dir dir, Epi epi, int ch, const RaggedItem *items,
-                              const int *prefix, int nitems, int nblocks, Tile t,
-                              size_t lds, hipStream_t s)
+hipError_t launch_ragged_fast(Dir dir, Epi epi, int ch, bool fuse,
+                              const RaggedItem *items, const int *prefix, int nitems,
+                              int nblocks, Tile t, size_t lds, hipStream_t s)
 {
     if (nblocks <= 0 || nitems <= 0) return hipSuccess;
     switch (ch) {
-    case 1: return dispatch_r<1>(dir, epi, items, prefix, nitems, nblocks, t, lds, s);
-    case 2: return dispatch_r<2>(dir, epi, items, prefix, nitems, nblocks, t, lds, s);
-    case 4: return dispatch_r<4>(dir, epi, items, prefix, nitems, nblocks, t, lds, s);
-    case 8: return dispatch_r<8>(dir, epi, items, prefix, nitems, nblocks, t, lds, s);
-    case 16: return dispatch_r<16>(dir, epi, items, prefix, nitems, nblocks, t, lds, s);
-    case 32: return dispatch_r<32>(dir, epi, items, prefix, nitems, nblocks, t, lds, s);
+    case 1: return dispatch_r<1>(dir, epi, fuse, items, prefix, nitems, nblocks, t, lds, s);
+    case 2: return dispatch_r<2>(dir, epi, fuse, items, prefix, nitems, nblocks, t, lds, s);
+    case 4: return dispatch_r<4>(dir, epi, fuse, items, prefix, nitems, nblocks, t, lds, s);
+    case 8: return dispatch_r<8>(dir, epi, fuse, items, prefix, nitems, nblocks, t, lds, s);
+    case 16: return dispatch_r<16>(dir, epi, fuse, items, prefix, nitems, nblocks, t, lds, s);
+    case 32: return dispatch_r<32>(dir, epi, fuse, items, prefix, nitems, nblocks, t, lds, s);
     }
     return hipErrorInvalidValue;
 }

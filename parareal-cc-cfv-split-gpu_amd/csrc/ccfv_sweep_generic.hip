@@ -13,10 +13,9 @@ namespace ccfv {
 // cs: the line in LDS (pidx layout), overwritten with the result.
 // aux: a line-sized LDS buffer owned by this wave.
 template <int DIR, int BC, int CH, bool POST>
-__device__ void process_line(const SweepArgs &a, double *cs, double *aux,
-                             int line, int lane, int field)
+__device__ void process_line(const SweepArgs &a, const Times tt, double *cs, double *aux,
+                             int line, int lane)
 {
-    const Times tt = sub_times(a, field);
     const int n = a.n;
     const int NL = n / CH;
     const bool active = lane < NL;
@@ -60,8 +59,11 @@ __device__ void process_line(const SweepArgs &a, double *cs, double *aux,
                               t + 4 * n, t + 4 * n + 64, cs);
 }
 
-// one block: field `field` of the launch, column tile `tile` (W lines)
-template <int BC, int CH, bool POST, int EPI>
+// one block: field `field` of the launch, column tile `tile` (W lines).
+// FUSE: after the sweep (and its epilogue, whose result stays in LDS) the
+// same lines take the next step's first x sub-step (times from field0_2 and
+// jdt2, the same step size) and go to out2.
+template <int BC, int CH, bool POST, int EPI, bool FUSE>
 __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, int field, int tile,
                                              int W, int NW, double *lds)
 {
@@ -72,11 +74,24 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, int field, int 
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double *aux = lds + (W + wave) * LP;
+    const Times tt = sub_times(a, field);
     for (int l = wave; l < W; l += NW)
-        process_line<DIR_X, BC, CH, POST>(a, lds + l * LP, aux, j0 + l, lane, field);
+        process_line<DIR_X, BC, CH, POST>(a, tt, lds + l * LP, aux, j0 + l, lane);
     __syncthreads();
-    if (EPI == EPI_NONE) store_tile_x(a.out + foff, a.ld, n, j0, W, lds, LP);
-    else epilogue_x<EPI>(a, n, j0, W, lds, LP, tile);
+    if (!FUSE) {
+        if (EPI == EPI_NONE) store_tile_x(a.out + foff, a.ld, n, j0, W, lds, LP);
+        else epilogue_x<EPI>(a, n, j0, W, lds, LP, tile);
+        return;
+    }
+    if (EPI != EPI_NONE) {
+        epilogue_x<EPI, true>(a, n, j0, W, lds, LP, tile);
+        __syncthreads();
+    }
+    const Times t2 = sub_times_next(a, field);
+    for (int l = wave; l < W; l += NW)
+        process_line<DIR_X, BC, CH, false>(a, t2, lds + l * LP, aux, j0 + l, lane);
+    __syncthreads();
+    store_tile_x(a.out2 + foff, a.ld, n, j0, W, lds, LP);
 }
 
 template <int BC, int CH>
@@ -90,18 +105,19 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs &a, int field, int 
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double *aux = lds + (W + wave) * LP;
+    const Times tt = sub_times(a, field);
     for (int l = wave; l < W; l += NW)
-        process_line<DIR_Y, BC, CH, false>(a, lds + l * LP, aux, i0 + l, lane, field);
+        process_line<DIR_Y, BC, CH, false>(a, tt, lds + l * LP, aux, i0 + l, lane);
     __syncthreads();
     store_tile_y(a.out + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
 }
 
-template <int BC, int CH, bool POST, int EPI>
+template <int BC, int CH, bool POST, int EPI, bool FUSE>
 __global__ __launch_bounds__(512) void sweep_x_kernel(SweepArgs a, int W, int NW)
 {
     extern __shared__ double lds[];
-    sweep_x_body<BC, CH, POST, EPI>(a, blockIdx.y, xcd_tile(blockIdx.x, gridDim.x), W,
-                                    NW, lds);
+    sweep_x_body<BC, CH, POST, EPI, FUSE>(a, blockIdx.y, xcd_tile(blockIdx.x, gridDim.x),
+                                          W, NW, lds);
 }
 
 template <int BC, int CH>
@@ -111,7 +127,7 @@ __global__ __launch_bounds__(512) void sweep_y_kernel(SweepArgs a, int W, int NW
     sweep_y_body<BC, CH>(a, blockIdx.y, blockIdx.x, W, NW, lds);
 }
 
-template <int BC, int CH, bool POST, int EPI>
+template <int BC, int CH, bool POST, int EPI, bool FUSE>
 __global__ __launch_bounds__(512) void ragged_x_kernel(const RaggedItem *items,
                                                        const int *prefix, int nitems,
                                                        int W, int NW)
@@ -125,7 +141,7 @@ __global__ __launch_bounds__(512) void ragged_x_kernel(const RaggedItem *items,
     // the arguments stay in global memory (scalar-cached); a private copy of
     // the struct trips the gfx950 backend for this kernel
     const SweepArgs &a = items[i].a;
-    sweep_x_body<BC, CH, POST, EPI>(a, field, local - field * nt, W, NW, lds);
+    sweep_x_body<BC, CH, POST, EPI, FUSE>(a, field, local - field * nt, W, NW, lds);
 }
 
 template <int BC, int CH>
@@ -164,48 +180,56 @@ static hipError_t launch(K k, dim3 grid, dim3 block, size_t lds, hipStream_t s,
     return hipGetLastError();
 }
 
+template <int BC, int CH, bool POST, bool FUSE>
+static hipError_t dispatch_x(Epi epi, dim3 grid, dim3 block, size_t lds, hipStream_t s,
+                             const SweepArgs &a, Tile t)
+{
+    if (epi == EPI_NONE)
+        return launch(sweep_x_kernel<BC, CH, POST, EPI_NONE, FUSE>, grid, block, lds, s, a,
+                      t);
+    if (epi == EPI_INIT)
+        return launch(sweep_x_kernel<BC, CH, POST, EPI_INIT, FUSE>, grid, block, lds, s, a,
+                      t);
+    return launch(sweep_x_kernel<BC, CH, POST, EPI_CORRECT, FUSE>, grid, block, lds, s, a,
+                  t);
+}
+
 template <int BC, int CH>
-static hipError_t dispatch(Dir dir, bool post, Epi epi, const SweepArgs &a,
+static hipError_t dispatch(Dir dir, bool post, Epi epi, bool fuse, const SweepArgs &a,
                            int nf, Tile t, hipStream_t s)
 {
     dim3 grid(a.nl / t.w, nf), block(64 * t.nw);
     size_t lds = sweep_lds_bytes_generic(a.n, t);
     if (dir == DIR_Y) return launch(sweep_y_kernel<BC, CH>, grid, block, lds, s, a, t);
-    if (BC == BC_REF && post) {
-        if (epi == EPI_NONE)
-            return launch(sweep_x_kernel<BC, CH, true, EPI_NONE>, grid, block, lds, s, a, t);
-        if (epi == EPI_INIT)
-            return launch(sweep_x_kernel<BC, CH, true, EPI_INIT>, grid, block, lds, s, a, t);
-        return launch(sweep_x_kernel<BC, CH, true, EPI_CORRECT>, grid, block, lds, s, a, t);
-    }
-    if (epi == EPI_NONE)
-        return launch(sweep_x_kernel<BC, CH, false, EPI_NONE>, grid, block, lds, s, a, t);
-    if (epi == EPI_INIT)
-        return launch(sweep_x_kernel<BC, CH, false, EPI_INIT>, grid, block, lds, s, a, t);
-    return launch(sweep_x_kernel<BC, CH, false, EPI_CORRECT>, grid, block, lds, s, a, t);
+    if (BC == BC_REF && post)
+        return fuse ? dispatch_x<BC, CH, true, true>(epi, grid, block, lds, s, a, t)
+                    : dispatch_x<BC, CH, true, false>(epi, grid, block, lds, s, a, t);
+    return fuse ? dispatch_x<BC, CH, false, true>(epi, grid, block, lds, s, a, t)
+                : dispatch_x<BC, CH, false, false>(epi, grid, block, lds, s, a, t);
 }
 
 template <int BC>
-static hipError_t dispatch_bc(Dir dir, bool post, Epi epi, const SweepArgs &a,
+static hipError_t dispatch_bc(Dir dir, bool post, Epi epi, bool fuse, const SweepArgs &a,
                               int nf, Tile t, hipStream_t s)
 {
     switch (chunk_for(a.n)) {
-    case 1: return dispatch<BC, 1>(dir, post, epi, a, nf, t, s);
-    case 2: return dispatch<BC, 2>(dir, post, epi, a, nf, t, s);
-    case 4: return dispatch<BC, 4>(dir, post, epi, a, nf, t, s);
-    case 8: return dispatch<BC, 8>(dir, post, epi, a, nf, t, s);
-    case 16: return dispatch<BC, 16>(dir, post, epi, a, nf, t, s);
-    case 32: return dispatch<BC, 32>(dir, post, epi, a, nf, t, s);
+    case 1: return dispatch<BC, 1>(dir, post, epi, fuse, a, nf, t, s);
+    case 2: return dispatch<BC, 2>(dir, post, epi, fuse, a, nf, t, s);
+    case 4: return dispatch<BC, 4>(dir, post, epi, fuse, a, nf, t, s);
+    case 8: return dispatch<BC, 8>(dir, post, epi, fuse, a, nf, t, s);
+    case 16: return dispatch<BC, 16>(dir, post, epi, fuse, a, nf, t, s);
+    case 32: return dispatch<BC, 32>(dir, post, epi, fuse, a, nf, t, s);
     }
     return hipErrorInvalidValue;
 }
 
 hipError_t launch_sweep_generic(Dir dir, int bc, bool post, Epi epi,
                                 const SweepArgs &a, int nfields, Tile tile,
-                                hipStream_t s)
+                                hipStream_t s, bool fuse)
 {
-    if (bc == BC_PER) return dispatch_bc<BC_PER>(dir, false, epi, a, nfields, tile, s);
-    return dispatch_bc<BC_REF>(dir, post, epi, a, nfields, tile, s);
+    if (bc == BC_PER)
+        return dispatch_bc<BC_PER>(dir, false, epi, fuse, a, nfields, tile, s);
+    return dispatch_bc<BC_REF>(dir, post, epi, fuse, a, nfields, tile, s);
 }
 
 template <typename K>
@@ -224,53 +248,60 @@ static hipError_t launch_r(K k, int nblocks, Tile t, size_t lds, hipStream_t s,
     return hipGetLastError();
 }
 
+template <int BC, int CH, bool POST, bool FUSE>
+static hipError_t dispatch_rx(Epi epi, const RaggedItem *it, const int *pf, int ni, int nb,
+                              Tile t, size_t lds, hipStream_t s)
+{
+    if (epi == EPI_NONE)
+        return launch_r(ragged_x_kernel<BC, CH, POST, EPI_NONE, FUSE>, nb, t, lds, s, it, pf,
+                        ni);
+    if (epi == EPI_INIT)
+        return launch_r(ragged_x_kernel<BC, CH, POST, EPI_INIT, FUSE>, nb, t, lds, s, it, pf,
+                        ni);
+    return launch_r(ragged_x_kernel<BC, CH, POST, EPI_CORRECT, FUSE>, nb, t, lds, s, it, pf,
+                    ni);
+}
+
 template <int BC, int CH>
-static hipError_t dispatch_r(Dir dir, bool post, Epi epi, const RaggedItem *it,
+static hipError_t dispatch_r(Dir dir, bool post, Epi epi, bool fuse, const RaggedItem *it,
                              const int *pf, int ni, int nb, Tile t, size_t lds,
                              hipStream_t s)
 {
     if (dir == DIR_Y) return launch_r(ragged_y_kernel<BC, CH>, nb, t, lds, s, it, pf, ni);
-    if (BC == BC_REF && post) {
-        if (epi == EPI_NONE)
-            return launch_r(ragged_x_kernel<BC, CH, true, EPI_NONE>, nb, t, lds, s, it, pf, ni);
-        if (epi == EPI_INIT)
-            return launch_r(ragged_x_kernel<BC, CH, true, EPI_INIT>, nb, t, lds, s, it, pf, ni);
-        return launch_r(ragged_x_kernel<BC, CH, true, EPI_CORRECT>, nb, t, lds, s, it, pf, ni);
-    }
-    if (epi == EPI_NONE)
-        return launch_r(ragged_x_kernel<BC, CH, false, EPI_NONE>, nb, t, lds, s, it, pf, ni);
-    if (epi == EPI_INIT)
-        return launch_r(ragged_x_kernel<BC, CH, false, EPI_INIT>, nb, t, lds, s, it, pf, ni);
-    return launch_r(ragged_x_kernel<BC, CH, false, EPI_CORRECT>, nb, t, lds, s, it, pf, ni);
+    if (BC == BC_REF && post)
+        return fuse ? dispatch_rx<BC, CH, true, true>(epi, it, pf, ni, nb, t, lds, s)
+                    : dispatch_rx<BC, CH, true, false>(epi, it, pf, ni, nb, t, lds, s);
+    return fuse ? dispatch_rx<BC, CH, false, true>(epi, it, pf, ni, nb, t, lds, s)
+                : dispatch_rx<BC, CH, false, false>(epi, it, pf, ni, nb, t, lds, s);
 }
 
 template <int BC>
-static hipError_t dispatch_rbc(Dir dir, bool post, Epi epi, int ch, const RaggedItem *it,
-                               const int *pf, int ni, int nb, Tile t, size_t lds,
-                               hipStream_t s)
+static hipError_t dispatch_rbc(Dir dir, bool post, Epi epi, bool fuse, int ch,
+                               const RaggedItem *it, const int *pf, int ni, int nb, Tile t,
+                               size_t lds, hipStream_t s)
 {
     switch (ch) {
-    case 1: return dispatch_r<BC, 1>(dir, post, epi, it, pf, ni, nb, t, lds, s);
-    case 2: return dispatch_r<BC, 2>(dir, post, epi, it, pf, ni, nb, t, lds, s);
-    case 4: return dispatch_r<BC, 4>(dir, post, epi, it, pf, ni, nb, t, lds, s);
-    case 8: return dispatch_r<BC, 8>(dir, post, epi, it, pf, ni, nb, t, lds, s);
-    case 16: return dispatch_r<BC, 16>(dir, post, epi, it, pf, ni, nb, t, lds, s);
-    case 32: return dispatch_r<BC, 32>(dir, post, epi, it, pf, ni, nb, t, lds, s);
+    case 1: return dispatch_r<BC, 1>(dir, post, epi, fuse, it, pf, ni, nb, t, lds, s);
+    case 2: return dispatch_r<BC, 2>(dir, post, epi, fuse, it, pf, ni, nb, t, lds, s);
+    case 4: return dispatch_r<BC, 4>(dir, post, epi, fuse, it, pf, ni, nb, t, lds, s);
+    case 8: return dispatch_r<BC, 8>(dir, post, epi, fuse, it, pf, ni, nb, t, lds, s);
+    case 16: return dispatch_r<BC, 16>(dir, post, epi, fuse, it, pf, ni, nb, t, lds, s);
+    case 32: return dispatch_r<BC, 32>(dir, post, epi, fuse, it, pf, ni, nb, t, lds, s);
     }
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_ragged_generic(Dir dir, int bc, bool post, Epi epi, int ch,
+hipError_t launch_ragged_generic(Dir dir, int bc, bool post, Epi epi, int ch, bool fuse,
                                  const RaggedItem *items, const int *prefix,
                                  int nitems, int nblocks, Tile t, size_t lds,
                                  hipStream_t s)
 {
     if (nblocks <= 0 || nitems <= 0) return hipSuccess;
     if (bc == BC_PER)
-        return dispatch_rbc<BC_PER>(dir, false, epi, ch, items, prefix, nitems, nblocks,
-                                    t, lds, s);
-    return dispatch_rbc<BC_REF>(dir, post, epi, ch, items, prefix, nitems, nblocks, t,
-                                lds, s);
+        return dispatch_rbc<BC_PER>(dir, false, epi, fuse, ch, items, prefix, nitems,
+                                    nblocks, t, lds, s);
+    return dispatch_rbc<BC_REF>(dir, post, epi, fuse, ch, items, prefix, nitems, nblocks,
+                                t, lds, s);
 }
 
 }  // namespace ccfv

@@ -71,3 +71,35 @@ def test_not_converged_status(gpu):
         with pytest.raises(gpu.CcfvError) as ei:
             c.parareal_solve(0.0, 0.5, 64, 8, 2, 1e-15, raise_on_notconv=True)
         assert ei.value.code == T.CCFV_ENOTCONV
+
+
+@pytest.mark.parametrize("make", [
+    lambda: (T.rotation_problem(64), T.rotation_ic()),
+    lambda: (T.reference_problem(32, 0.01), T.reference_ic()),
+    lambda: (T.swirl_problem(32, T=1.0), T.cosine_bell_ic()),
+])
+def test_fused_x_launches_change_nothing(gpu, make):
+    """The last x sweep of a step and the first of the next as one launch
+    (default) against one launch per sweep: bitwise equal, for the plain
+    solve, the fine propagator and both Parareal schedules."""
+    p, ic = make()
+    out = {}
+    for fuse in (0, 1):
+        with gpu.Context(p) as c:
+            c.set_option(T.OPT_FUSE, fuse)
+            assert c.get_option(T.OPT_FUSE) == fuse
+            c.fill_exact(ic, 0.0)
+            c.plain_solve(0.0, 0.25, 20)
+            plain = c.get_state()
+            c.fine(0.25, 0.125, 5)
+            fine = c.get_state()
+            res = [plain, fine]
+            for pipe in (0, 1):
+                c.set_option(T.OPT_PIPELINE, pipe)
+                c.fill_exact(ic, 0.0)
+                rep = c.parareal_solve(0.0, 0.5, 64, 8, 4, 0.0)
+                res.append(c.get_state())
+                res.append(np.array(rep.increments))
+            out[fuse] = res
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
