@@ -71,6 +71,47 @@ __host__ __device__ inline double face_foot(const VelParams &v, double lo,
     }
 }
 
+// The swirl velocity is separable: along a line only sin^2(pi s) changes;
+// sin(2 pi o) at the line's two corner coordinates and cos(pi t / T) at the
+// sub-step's two sampling times are line constants.  face_foot_swirl
+// evaluates face_foot for CCFV_VEL_SWIRL with them, multiplying in the
+// order of vel_x / vel_y, so the foot is bitwise the same.
+struct SwirlLine {
+    double a0, a1;    // sin(2 pi o0), sin(2 pi o1)
+    double chi, clo;  // cos(pi t_hi / T), cos(pi t_lo / T)
+};
+
+__host__ __device__ inline SwirlLine swirl_line(const VelParams &v, double o0, double o1,
+                                                double t_hi, double t_lo)
+{
+#pragma clang fp contract(off)
+    SwirlLine l;
+    l.a0 = sin(2.0 * M_PI * o0);
+    l.a1 = sin(2.0 * M_PI * o1);
+    l.chi = cos(M_PI * t_hi / v.v0);
+    l.clo = cos(M_PI * t_lo / v.v0);
+    return l;
+}
+
+template <int DIR>
+__host__ __device__ inline double face_foot_swirl(const SwirlLine &l, double lo, double d,
+                                                  double tau, int q)
+{
+#pragma clang fp contract(off)
+    double s = lo + d * q;
+    double ht = 0.5 * tau;
+    // sign: u = +sin^2(pi x)..., v = -sin^2(pi y)...
+    auto sq = [](double p) { return DIR == 0 ? p * p : -(p * p); };
+    const double ps = sq(sin(M_PI * s));
+    double v0 = ps * l.a0 * l.chi;
+    double v1 = sq(sin(M_PI * (s - tau * v0))) * l.a0 * l.clo;
+    double f0 = s - ht * (v0 + v1);
+    double w0 = ps * l.a1 * l.chi;
+    double w1 = sq(sin(M_PI * (s - tau * w0))) * l.a1 * l.clo;
+    double f1 = s - ht * (w0 + w1);
+    return 0.5 * (f0 + f1);
+}
+
 // cell of position x and the offset inside it: floor((x - lo)/d)
 // (T2S4_Parareal.cu:325-328); n bounds the int conversion
 __host__ __device__ inline int cell_of(double lo, double d, int n, double x,

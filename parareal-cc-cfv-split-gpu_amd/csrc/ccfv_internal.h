@@ -71,7 +71,7 @@ struct SweepArgs {
     // field offset and time offset), whose result goes to out2
     double *out2;
     int field0_2;
-    double jdt2;
+    double t0_2, jdt2;
 };
 
 struct Tile {

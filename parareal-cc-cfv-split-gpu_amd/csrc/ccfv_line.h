@@ -54,11 +54,11 @@ __device__ __forceinline__ Times sub_times(const SweepArgs &a, int f)
 }
 
 // sampling times of the first x sub-step of the next step in a fused x
-// launch: the arithmetic of sub_times with field0_2 and jdt2
+// launch: the arithmetic of sub_times with t0_2, field0_2 and jdt2
 __device__ __forceinline__ Times sub_times_next(const SweepArgs &a, int f)
 {
 #pragma clang fp contract(off)
-    double tn = a.t0 + (double)(a.field0_2 + f) * a.tDT;
+    double tn = a.t0_2 + (double)(a.field0_2 + f) * a.tDT;
     double tl = tn + a.jdt2;
     Times t;
     t.lo = tl;
@@ -66,13 +66,33 @@ __device__ __forceinline__ Times sub_times_next(const SweepArgs &a, int f)
     return t;
 }
 
-// foot of face q of the line with corner coordinates o0, o1
+// a line's corner coordinates in the other direction and, for the swirl,
+// its velocity constants
+struct LineGeo {
+    double o0, o1;
+    SwirlLine sw;
+};
+
+__device__ __forceinline__ LineGeo line_geo(const SweepArgs &a, const Times &t, int line)
+{
+    LineGeo g;
+    g.o0 = corner(a.olo, a.od, line);
+    g.o1 = corner(a.olo, a.od, line + 1);
+    if (a.vel_kind == CCFV_VEL_SWIRL) {
+        VelParams v{a.vel_kind, a.v0, a.v1, a.v2};
+        g.sw = swirl_line(v, g.o0, g.o1, t.hi, t.lo);
+    }
+    return g;
+}
+
+// foot of face q of the line
 template <int DIR>
 __device__ __forceinline__ double face_foot(const SweepArgs &a, const Times &t,
-                                            int q, double o0, double o1)
+                                            int q, const LineGeo &lg)
 {
+    if (a.vel_kind == CCFV_VEL_SWIRL) return face_foot_swirl<DIR>(lg.sw, a.lo, a.d, a.tau, q);
     VelParams v{a.vel_kind, a.v0, a.v1, a.v2};
-    return face_foot<DIR>(v, a.lo, a.d, a.tau, t.hi, t.lo, q, o0, o1);
+    return face_foot<DIR>(v, a.lo, a.d, a.tau, t.hi, t.lo, q, lg.o0, lg.o1);
 }
 
 __device__ __forceinline__ int cell_of(const SweepArgs &a, double x, double &xi)
@@ -193,7 +213,7 @@ __device__ __forceinline__ double deriv_ref(const SweepArgs &a,
 template <int DIR, int BC, bool POST>
 __device__ __forceinline__ Face eval_face(const SweepArgs &a, const Times &tt,
                                           const double *cs, const double g[4],
-                                          int q, double o0, double o1,
+                                          int q, const LineGeo &lg,
                                           double inv12d)
 {
     Face f;
@@ -201,7 +221,7 @@ __device__ __forceinline__ Face eval_face(const SweepArgs &a, const Times &tt,
     if (BC == BC_PER) {
         // face n is face 0 one period on (exact periodicity of the fluxes)
         int qq = q == n ? 0 : q;
-        f.xb = face_foot<DIR>(a, tt, qq, o0, o1);
+        f.xb = face_foot<DIR>(a, tt, qq, lg);
         double xi;
         int m = cell_of(a, f.xb, xi);
         int mw = wrapn(m, n);
@@ -215,7 +235,7 @@ __device__ __forceinline__ Face eval_face(const SweepArgs &a, const Times &tt,
         f.mL = f.mR = q == n ? m + n : m;
         f.D = POST ? 0.0 : deriv(w, xi, inv12d);
     } else {
-        f.xb = face_foot<DIR>(a, tt, q, o0, o1);
+        f.xb = face_foot<DIR>(a, tt, q, lg);
         double xl, xr;
         {
 #pragma clang fp contract(off)
@@ -294,17 +314,15 @@ __device__ __forceinline__ double remap_cell(const SweepArgs &a,
 template <int DIR, int BC>
 __device__ void rhs_general(const SweepArgs &a, const Times &tt, const double *cs,
                             const double g[4], double *aux, int base, int ch,
-                            double o0, double o1, double inv12d, bool post)
+                            const LineGeo &lg, double inv12d, bool post)
 {
     bool bad = false;
-    Face fl = post ? eval_face<DIR, BC, true>(a, tt, cs, g, base, o0, o1, inv12d)
-                   : eval_face<DIR, BC, false>(a, tt, cs, g, base, o0, o1, inv12d);
+    Face fl = post ? eval_face<DIR, BC, true>(a, tt, cs, g, base, lg, inv12d)
+                   : eval_face<DIR, BC, false>(a, tt, cs, g, base, lg, inv12d);
 #pragma unroll 1
     for (int k = 0; k < ch; k++) {
-        Face fr = post ? eval_face<DIR, BC, true>(a, tt, cs, g, base + k + 1, o0,
-                                                  o1, inv12d)
-                       : eval_face<DIR, BC, false>(a, tt, cs, g, base + k + 1, o0,
-                                                   o1, inv12d);
+        Face fr = post ? eval_face<DIR, BC, true>(a, tt, cs, g, base + k + 1, lg, inv12d)
+                       : eval_face<DIR, BC, false>(a, tt, cs, g, base + k + 1, lg, inv12d);
         double cbar = remap_cell<BC>(a, cs, fl, fr, bad);
         aux[pidx(base + k)] = post ? cbar : cbar + a.coef * (fr.D - fl.D);
         fl = fr;

@@ -257,7 +257,7 @@ StageItem stage_item(ccfv_ctx *c, Dir dir, int substep, const SolveTable &T,
 bool fusable(const StepDesc &d, const StepDesc &n)
 {
     const double *res = d.epi == EPI_NONE ? d.out : d.u_next;
-    return n.in == res && n.h == d.h && n.nf == d.nf && n.tDT == d.tDT && n.t0 == d.t0;
+    return n.in == res && n.h == d.h && n.nf == d.nf && n.tDT == d.tDT;
 }
 
 }  // namespace
@@ -296,6 +296,7 @@ int emit_runs(Phase &ph, const std::vector<StepRun> &runs, int r0, int r1)
                         x3.fuse = true;
                         x3.item.a.out2 = n.out;
                         x3.item.a.field0_2 = n.field0;
+                        x3.item.a.t0_2 = n.t0;
                         x3.item.a.jdt2 = n.jdt;
                     } else {
                         const SolveTable *tn;

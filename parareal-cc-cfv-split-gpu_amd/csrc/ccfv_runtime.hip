@@ -442,11 +442,12 @@ int run_steps(ccfv_ctx *c, int count, const std::function<StepDesc(int)> &desc,
         StepDesc N = desc(i + 1);
         const double *res = D.epi == EPI_NONE ? D.out : D.u_next;
         const bool fuse = c->fuse && N.in == res && N.h == D.h && N.nf == D.nf &&
-                          N.tDT == D.tDT && N.t0 == D.t0;
+                          N.tDT == D.tDT;
         if (pre && (rc = pre(i + 1))) return rc;
         if (fuse) {
             x3.out2 = N.out;
             x3.field0_2 = N.field0;
+            x3.t0_2 = N.t0;
             x3.jdt2 = N.jdt;
             HIPCHK(timed_sweep_fused(c, x_kind(D.epi, batch), x_bytes(c, D.epi, D.nf, true),
                                      postflag, D.epi, x3, D.nf, xt));

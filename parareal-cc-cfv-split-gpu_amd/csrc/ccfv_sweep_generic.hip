@@ -21,12 +21,12 @@ __device__ void process_line(const SweepArgs &a, const Times tt, double *cs, dou
     const bool active = lane < NL;
     const int base = lane * CH;
     const double inv12d = 1.0 / (12.0 * a.d);
-    const double o0 = corner(a.olo, a.od, line), o1 = corner(a.olo, a.od, line + 1);
+    const LineGeo lg = line_geo(a, tt, line);
     double g[4] = {0.0, 0.0, 0.0, 0.0};
     if (BC == BC_REF) ghosts(cs, n, g);
 
     double b[CH];
-    if (active) rhs_general<DIR, BC>(a, tt, cs, g, aux, base, CH, o0, o1, inv12d, POST);
+    if (active) rhs_general<DIR, BC>(a, tt, cs, g, aux, base, CH, lg, inv12d, POST);
     if (POST) {
         // the reference's third sub-step takes the derivative of the remapped
         // line (T2S4_Parareal.cu:929-932)
@@ -34,12 +34,12 @@ __device__ void process_line(const SweepArgs &a, const Times tt, double *cs, dou
         double gs[4] = {0.0, 0.0, 0.0, 0.0};
         if (BC == BC_REF) ghosts(aux, n, gs);
         if (active) {
-            double dl = deriv_ref<BC>(a, aux, gs, face_foot<DIR>(a, tt, base, o0, o1),
+            double dl = deriv_ref<BC>(a, aux, gs, face_foot<DIR>(a, tt, base, lg),
                                       inv12d);
 #pragma unroll
             for (int k = 0; k < CH; k++) {
                 double dr = deriv_ref<BC>(a, aux, gs,
-                                          face_foot<DIR>(a, tt, base + k + 1, o0, o1),
+                                          face_foot<DIR>(a, tt, base + k + 1, lg),
                                           inv12d);
                 b[k] = aux[pidx(base + k)] + a.coef * (dr - dl);
                 dl = dr;
