@@ -178,6 +178,11 @@ int ccfv_coarse(ccfv_ctx *ctx, double tn, double DT);
 int ccfv_fine(ccfv_ctx *ctx, double tn, double DT, int32_t J);
 /* Plain serial-in-time solve: nt steps of (t1-t0)/nt from t0 (:1224-1243). */
 int ccfv_plain_solve(ccfv_ctx *ctx, double t0, double t1, int32_t nt);
+/* Steps n = n_begin .. n_end-1 of the serial fine loop, step n from
+ * t_origin + n*dt with step size dt, as the reference's loop computes them
+ * (:1234-1236); lets a caller stop every T_span steps for output. */
+int ccfv_advance(ccfv_ctx *ctx, double t_origin, double dt, int32_t n_begin,
+                 int32_t n_end);
 
 /* Parareal over [t0, t1]: n_slices coarse steps of DT=(t1-t0)/n_slices, fine
  * propagator J = nt_fine/n_slices steps of DT/J; iterations

@@ -58,6 +58,8 @@ _SIGS = {
                                  ctypes.c_double, ctypes.c_int32]),
     "ccfv_plain_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_int32]),
+    "ccfv_advance": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_int32, ctypes.c_int32]),
     "ccfv_parareal_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_int32,
                                            ctypes.c_int32, ctypes.c_int32,
@@ -236,6 +238,9 @@ class Context:
 
     def plain_solve(self, t0: float, t1: float, nt: int):
         _check(load().ccfv_plain_solve(self._h, t0, t1, nt), "ccfv_plain_solve")
+
+    def advance(self, t_origin: float, dt: float, n_begin: int, n_end: int):
+        _check(load().ccfv_advance(self._h, t_origin, dt, n_begin, n_end), "ccfv_advance")
 
     def parareal_solve(self, t0, t1, nt_fine, n_slices, kmax, tol=0.0,
                        raise_on_notconv=False) -> ParaReport:

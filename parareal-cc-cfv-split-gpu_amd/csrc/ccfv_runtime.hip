@@ -802,6 +802,21 @@ int ccfv_fine(ccfv_ctx *c, double tn, double DT, int32_t J)
     return check_flag(c);
 }
 
+int ccfv_advance(ccfv_ctx *c, double t_origin, double dt, int32_t n_begin, int32_t n_end)
+{
+    if (!c) return fail(CCFV_EINVAL, "NULL context");
+    if (n_begin < 0 || n_end < n_begin) return fail(CCFV_EINVAL, "need 0 <= n_begin <= n_end");
+    if (!(dt > 0.0) || !std::isfinite(dt) || !std::isfinite(t_origin))
+        return fail(CCFV_EINVAL, "step size must be finite and > 0");
+    c->cur = c->stream;
+    int rc = run_steps(c, n_end - n_begin, [&](int i) {
+        const int n = n_begin + i;
+        return StepDesc{c->state, c->state, 1, t_origin + n * dt, 0.0, 0, 0.0, dt};
+    });
+    if (rc) return rc;
+    return check_flag(c);
+}
+
 int ccfv_plain_solve(ccfv_ctx *c, double t0, double t1, int32_t nt)
 {
     // serial fine loop (T2S4_Parareal.cu:1060, :1234-1236)
