@@ -120,3 +120,31 @@ def test_full_size_rotation_plain_matches_oracle(gpu, oracle):
         g = c.get_state()
     o = oracle.plain(p, u0, 0.0, 1.0, 1024)
     assert rel_l2(g, o) <= 1e-12
+
+
+def test_plain_method_convergence_on_gpu(gpu, oracle):
+    """The plain-method workload (periodic sin*sin, velocity (1, 0.5),
+    nu = 0.01, CFL 0.8, T = 1) on 32^2..256^2 against the exact decaying
+    translated solution: third order at CFL 0.8 (the fourth-order remap
+    error accumulates over O(1/dx) steps), fourth order in space at a fixed
+    step count; the GPU errors equal the oracle's."""
+    def err(n, nt, dev):
+        p, ic = T.advection_sinsin_problem(n), T.advection_sinsin_ic()
+        ex = oracle.exact(p, ic, 1.0)
+        if dev:
+            with gpu.Context(p) as c:
+                c.fill_exact(ic, 0.0)
+                c.plain_solve(0.0, 1.0, nt)
+                u = c.get_state()
+        else:
+            u = oracle.plain(p, oracle.exact(p, ic, 0.0), 0.0, 1.0, nt)
+        return math.sqrt(float(np.mean((u - ex) ** 2)))
+    ns = (32, 64, 128, 256)
+    e = [err(n, T.advection_sinsin_steps(n), True) for n in ns]
+    orders = [math.log2(e[i] / e[i + 1]) for i in range(3)]
+    assert all(2.8 < o < 3.4 for o in orders), orders
+    for n, eg in zip(ns[:2], e[:2]):
+        eo = err(n, T.advection_sinsin_steps(n), False)
+        assert abs(eg - eo) <= 1e-9 * eo, (n, eg, eo)
+    es = [err(n, 2560, True) for n in (32, 64)]
+    assert 3.8 < math.log2(es[0] / es[1]) < 4.2
