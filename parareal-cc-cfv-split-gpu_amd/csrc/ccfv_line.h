@@ -541,33 +541,51 @@ __device__ __forceinline__ void store_tile_x(double *out, int ld, int n, int j0,
 //   EPI_INIT   : G cache <- g, U[n+1] <- g
 //   EPI_CORRECT: U[n+1] <- g + (F - G_old) (T2S4_Parareal.cu:779),
 //                G cache <- g, block partials of |dU|^2 and |U|^2
-// WB (fused launches): the corrected U[n+1] also replaces the tile in LDS,
-// where the next step's first x sub-step reads it.
-template <int EPI, bool WB = false>
-__device__ __forceinline__ void epilogue_x(const SweepArgs &a, int n, int j0,
-                                           int W, double *lds, int LP,
-                                           int slot)
+// The correction's three inputs for the first EPI_UNROLL elements of each
+// work-item, loaded before the sweep (they do not depend on it) so that the
+// epilogue of a one-field launch does not wait for memory.
+constexpr int EPI_UNROLL = 4;
+struct EpiInputs {
+    double2 go[EPI_UNROLL], fv[EPI_UNROLL], uo[EPI_UNROLL];
+};
+
+__device__ __forceinline__ void epi_load(const SweepArgs &a, int n, int j0, int W, int b0,
+                                         EpiInputs &in)
 {
     const int hw = W >> 1, sh = log2i(hw);
     const int total = hw * n, step = blockDim.x;
-    constexpr int U = 4;
+#pragma unroll
+    for (int u = 0; u < EPI_UNROLL; u++) {
+        const int idx = b0 + u * step;
+        if (idx < total) {
+            const int c2 = idx & (hw - 1), i = idx >> sh;
+            const size_t gi = (size_t)i * a.ld + j0 + 2 * c2;
+            in.go[u] = *reinterpret_cast<const double2 *>(a.g_cache + gi);
+            in.fv[u] = *reinterpret_cast<const double2 *>(a.f_in + gi);
+            in.uo[u] = *reinterpret_cast<const double2 *>(a.u_next + gi);
+        }
+    }
+}
+
+// WB (fused launches): the corrected U[n+1] also replaces the tile in LDS,
+// where the next step's first x sub-step reads it.  pre: the inputs of the
+// first chunk, loaded by epi_load(..., threadIdx.x, ...) (EPI_CORRECT).
+template <int EPI, bool WB = false>
+__device__ __forceinline__ void epilogue_x(const SweepArgs &a, int n, int j0,
+                                           int W, double *lds, int LP,
+                                           int slot, const EpiInputs *pre = nullptr)
+{
+    const int hw = W >> 1, sh = log2i(hw);
+    const int total = hw * n, step = blockDim.x;
+    constexpr int U = EPI_UNROLL;
     double s0 = 0.0, s1 = 0.0;
     for (int b0 = threadIdx.x; b0 < total; b0 += U * step) {
-        double2 go[U], fv[U], uo[U];
+        EpiInputs in;
         if (EPI == EPI_CORRECT) {
-            // the three inputs of U elements in flight together
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int idx = b0 + u * step;
-                if (idx < total) {
-                    const int c2 = idx & (hw - 1), i = idx >> sh;
-                    const size_t gi = (size_t)i * a.ld + j0 + 2 * c2;
-                    go[u] = *reinterpret_cast<const double2 *>(a.g_cache + gi);
-                    fv[u] = *reinterpret_cast<const double2 *>(a.f_in + gi);
-                    uo[u] = *reinterpret_cast<const double2 *>(a.u_next + gi);
-                }
-            }
+            if (pre && b0 == (int)threadIdx.x) in = *pre;
+            else epi_load(a, n, j0, W, b0, in);
         }
+        const double2 *go = in.go, *fv = in.fv, *uo = in.uo;
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int idx = b0 + u * step;

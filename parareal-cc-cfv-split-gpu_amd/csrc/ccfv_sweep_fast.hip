@@ -55,18 +55,36 @@ struct LaneTab<CH, false> {
     __device__ __forceinline__ double Z(int, int e) const { return t[3 * n + e]; }
 };
 
+// a line's cell shift and filter weights
+struct LineK {
+    int S;
+    double k0, k1, k2, k3, k4;
+};
+
+__device__ __forceinline__ LineK line_k(const SweepArgs &a, int line)
+{
+    LineK l;
+    l.S = (int)a.lineS[line];
+    const double *kp = a.lineK + 5 * (size_t)line;
+    l.k0 = kp[0];
+    l.k1 = kp[1];
+    l.k2 = kp[2];
+    l.k3 = kp[3];
+    l.k4 = kp[4];
+    return l;
+}
+
 template <int CH, bool REG>
 __device__ __forceinline__ void fast_line(const SweepArgs &a, double *cs,
-                                          int line, int lane,
+                                          const LineK &lk, int lane,
                                           const LaneTab<CH, REG> &tb, double mfl,
                                           double mbl, double pneg0)
 {
     const int n = a.n, NL = n / CH;
     const bool active = lane < NL;
     const int base = lane * CH;
-    const int S = (int)a.lineS[line];
-    const double *kp = a.lineK + 5 * (size_t)line;
-    const double k0 = kp[0], k1 = kp[1], k2 = kp[2], k3 = kp[3], k4 = kp[4];
+    const int S = lk.S;
+    const double k0 = lk.k0, k1 = lk.k1, k2 = lk.k2, k3 = lk.k3, k4 = lk.k4;
     double b[CH];
     if (active) {
         double cw[CH + 5];
@@ -173,6 +191,24 @@ __device__ __forceinline__ void fast_line(const SweepArgs &a, double *cs,
     }
 }
 
+// The wave's lines l0 = wave, wave+NW, ... of a tile whose first line
+// starts at line index L0; each line's parameters are loaded while the
+// previous line is swept.
+template <int CH, bool REG>
+__device__ __forceinline__ void sweep_lines(const SweepArgs &a, double *lds, int LP, int L0,
+                                            int W, int NW, int wave, int lane,
+                                            const LineK &first, const LaneTab<CH, REG> &tb,
+                                            double mfl, double mbl, double pneg0)
+{
+    LineK cur = first;
+    for (int l = wave; l < W; l += NW) {
+        LineK nxt = cur;
+        if (l + NW < W) nxt = line_k(a, L0 + l + NW);
+        fast_line<CH, REG>(a, lds + l * LP, cur, lane, tb, mfl, mbl, pneg0);
+        cur = nxt;
+    }
+}
+
 // FUSE: after the sweep (and its epilogue, whose result stays in LDS) the
 // same lines take the next step's first x sub-step, with the same table
 // (one step size) and the same per-line shift, and go to out2.
@@ -190,22 +226,25 @@ __device__ __forceinline__ void fast_x_body(const SweepArgs &a, int field, int t
     tb.load(a, lane * CH, lane < n / CH);
     const double mfl = a.tab[4 * n + lane], mbl = a.tab[4 * n + 64 + lane];
     const double pneg0 = a.tab[2 * n];
+    // the wave's first line parameters (and the correction's inputs) travel
+    // with the tile loads
+    const LineK first = line_k(a, j0 + wave);
+    EpiInputs pre;
+    if (EPI == EPI_CORRECT) epi_load(a, n, j0, W, threadIdx.x, pre);
     load_tile_x(a.in + foff, a.ld, n, j0, W, lds, LP);
     __syncthreads();
-    for (int l = wave; l < W; l += NW)
-        fast_line<CH, REG>(a, lds + l * LP, j0 + l, lane, tb, mfl, mbl, pneg0);
+    sweep_lines<CH, REG>(a, lds, LP, j0, W, NW, wave, lane, first, tb, mfl, mbl, pneg0);
     __syncthreads();
     if (!FUSE) {
         if (EPI == EPI_NONE) store_tile_x(a.out + foff, a.ld, n, j0, W, lds, LP);
-        else epilogue_x<EPI>(a, n, j0, W, lds, LP, tile);
+        else epilogue_x<EPI>(a, n, j0, W, lds, LP, tile, &pre);
         return;
     }
     if (EPI != EPI_NONE) {
-        epilogue_x<EPI, true>(a, n, j0, W, lds, LP, tile);
+        epilogue_x<EPI, true>(a, n, j0, W, lds, LP, tile, &pre);
         __syncthreads();
     }
-    for (int l = wave; l < W; l += NW)
-        fast_line<CH, REG>(a, lds + l * LP, j0 + l, lane, tb, mfl, mbl, pneg0);
+    sweep_lines<CH, REG>(a, lds, LP, j0, W, NW, wave, lane, first, tb, mfl, mbl, pneg0);
     __syncthreads();
     store_tile_x(a.out2 + foff, a.ld, n, j0, W, lds, LP);
 }
@@ -224,10 +263,10 @@ __device__ __forceinline__ void fast_y_body(const SweepArgs &a, int field, int t
     tb.load(a, lane * CH, lane < n / CH);
     const double mfl = a.tab[4 * n + lane], mbl = a.tab[4 * n + 64 + lane];
     const double pneg0 = a.tab[2 * n];
+    const LineK first = line_k(a, i0 + wave);
     load_tile_y(a.in + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
     __syncthreads();
-    for (int l = wave; l < W; l += NW)
-        fast_line<CH, REG>(a, lds + l * LP, i0 + l, lane, tb, mfl, mbl, pneg0);
+    sweep_lines<CH, REG>(a, lds, LP, i0, W, NW, wave, lane, first, tb, mfl, mbl, pneg0);
     __syncthreads();
     store_tile_y(a.out + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
 }
