@@ -131,3 +131,19 @@ def test_batch_not_converged_status(gpu):
         with pytest.raises(gpu.CcfvError) as ei:
             b.parareal(0.0, 0.5, 64, 8, 2, 1e-15, raise_on_notconv=True)
         assert ei.value.code == T.CCFV_ENOTCONV
+
+
+def test_one_failing_problem_leaves_the_others(gpu):
+    """A problem whose feet cross (swirl, one step of 1.0 at 64^2) makes the
+    batch report CCFV_ECFL; the other problems' results are those they get
+    alone."""
+    good = mix()[0]
+    bad = (T.swirl_problem(64, T=1.0), T.cosine_bell_ic())
+    with gpu.Batch([good[0], bad[0]]) as b:
+        b[0].fill_exact(good[1], 0.0)
+        b[1].fill_exact(bad[1], 0.0)
+        with pytest.raises(gpu.CcfvError) as ei:
+            b.plain([0.0, 0.0], [0.25, 1.0], [16, 1])
+        assert ei.value.code == T.CCFV_ECFL
+        got = b[0].get_state()
+    assert np.array_equal(got, solo_plain(gpu, good[0], good[1], 0.0, 0.25, 16))

@@ -148,3 +148,20 @@ def test_plain_method_convergence_on_gpu(gpu, oracle):
         assert abs(eg - eo) <= 1e-9 * eo, (n, eg, eo)
     es = [err(n, 2560, True) for n in (32, 64)]
     assert 3.8 < math.log2(es[0] / es[1]) < 4.2
+
+
+def test_crossed_feet_are_an_error_and_recoverable(gpu, oracle):
+    """A step so long that characteristic feet cross (swirl, h = 1 at 64^2:
+    the foot map is no longer monotone) reports CCFV_ECFL; the context stays
+    usable and the next valid step matches the oracle."""
+    p, ic = T.swirl_problem(64, T=1.0), T.cosine_bell_ic()
+    u0 = oracle.exact(p, ic, 0.0)
+    with gpu.Context(p) as c:
+        c.set_state(u0)
+        with pytest.raises(gpu.CcfvError) as ei:
+            c.step(0.0, 1.0)
+        assert ei.value.code == T.CCFV_ECFL
+        c.set_state(u0)
+        c.step(0.0, 0.01)
+        g = c.get_state()
+    assert rel_l2(g, oracle.step(p, u0, 0.0, 0.01)) <= 1e-13

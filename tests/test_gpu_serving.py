@@ -123,3 +123,20 @@ def test_open_loop_run_on_the_engine(gpu):
     assert all(r.latency > 0 for r in res.records)
     s = res.summary()
     assert s["p50_ms"] <= s["p99_ms"] <= s["p999_ms"]
+
+
+def test_failing_request_leaves_the_mix(gpu):
+    """A request whose coarse step crosses feet (swirl, DT = 1 at 64^2)
+    finishes with CCFV_ECFL; the requests beside it are bitwise as alone."""
+    mix = small_mix()[:3]
+    bad = workloads.Solve(T.swirl_problem(64, T=1.0), T.cosine_bell_ic(), 0.0, 2.0, 4, 2,
+                          kind="swirl-crossing")
+    ref = [alone(s, 3, 0.0) for s in mix]
+    with ccfv.Server(mem_budget=1 << 30, max_active=8) as srv:
+        ids = [srv.submit(s.request(3, 0.0)) for s in mix]
+        bid = srv.submit(bad.request(3, 0.0))
+        res = drain(srv)
+    assert res[bid][2] == T.CCFV_ECFL
+    for i, rid in enumerate(ids):
+        assert res[rid][2] == T.CCFV_OK
+        assert np.array_equal(res[rid][0], ref[i][0])
