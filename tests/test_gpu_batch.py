@@ -147,3 +147,22 @@ def test_one_failing_problem_leaves_the_others(gpu):
         assert ei.value.code == T.CCFV_ECFL
         got = b[0].get_state()
     assert np.array_equal(got, solo_plain(gpu, good[0], good[1], 0.0, 0.25, 16))
+
+
+@pytest.mark.parametrize("kmax,ns", [(0, 4), (1, 1), (2, 2)])
+def test_batch_small_plans(gpu, kmax, ns):
+    """kmax = 0 (the k=0 coarse sweep alone), a single slice, two slices:
+    every problem as alone, reports included."""
+    m = [mix()[0], mix()[1], (T.rotation_problem(8), T.rotation_ic())]
+    nt = [8 * ns, 4 * ns, 2 * ns]
+    with gpu.Batch([p for p, _ in m]) as b:
+        for i, (_, ic) in enumerate(m):
+            b[i].fill_exact(ic, 0.0)
+        reps = b.parareal(0.0, 0.25, nt, ns, kmax, 0.0)
+        got = [b[i].get_state() for i in range(len(m))]
+    for i, (p, ic) in enumerate(m):
+        ref, rr = solo_parareal(gpu, p, ic, 0.0, 0.25, nt[i], ns, kmax, 0.0)
+        assert np.array_equal(got[i], ref), i
+        assert reps[i].iterations == rr.iterations == min(kmax, ns)
+        assert reps[i].converged == rr.converged
+        assert reps[i].coarse_steps_run == rr.coarse_steps_run

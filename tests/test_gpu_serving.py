@@ -140,3 +140,24 @@ def test_failing_request_leaves_the_mix(gpu):
     for i, rid in enumerate(ids):
         assert res[rid][2] == T.CCFV_OK
         assert np.array_equal(res[rid][0], ref[i][0])
+
+
+def test_small_plans_and_grids(gpu):
+    """kmax = 0, one slice and an 8x8 grid through the engine, as alone."""
+    reqs = [
+        workloads.Solve(T.rotation_problem(8), T.rotation_ic(), 0.0, 0.25, 4, 4),
+        workloads.Solve(T.rotation_problem(64), T.rotation_ic(), 0.0, 0.25, 8, 1),
+        workloads.Solve(T.reference_problem(16, 0.01), T.reference_ic(), 0.0, 0.1, 4, 2),
+    ]
+    plans = [(0, 0.0), (1, 0.0), (2, 1e-12)]
+    with ccfv.Server(mem_budget=1 << 28, max_active=4) as srv:
+        ids = [srv.submit(s.request(k, tol)) for s, (k, tol) in zip(reqs, plans)]
+        res = drain(srv)
+    for s, (k, tol), rid in zip(reqs, plans, ids):
+        u, rep, st = res[rid]
+        with gpu.Context(s.problem) as c:
+            c.fill_exact(s.ic, s.t0)
+            rs = c.parareal_solve(s.t0, s.t1, s.nt_fine, s.n_slices, k, tol)
+            assert np.array_equal(c.get_state(), u)
+            assert rs.iterations == rep.iterations and rs.converged == rep.converged
+        assert st in (T.CCFV_OK, T.CCFV_ENOTCONV)
