@@ -275,7 +275,7 @@ int ccfv_profile_enable(ccfv_ctx *ctx, int32_t enable);
 int ccfv_profile_read(ccfv_ctx *ctx, ccfv_kernel_stat *out);
 
 /* ---- batches of independent problems -----------------------------------
- * (config-4 style workloads; no counterpart in the reference, which solves
+ * (workloads of many independent problems; no counterpart in the reference, which solves
  * one hard-coded problem).  Problems may differ in grid (multiples of 8),
  * velocity, diffusion, boundary mode, time interval, steps and slices.  Every
  * sub-step of every problem is issued through ragged launches (one per sweep
