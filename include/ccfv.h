@@ -317,14 +317,19 @@ typedef struct ccfv_server_config {
     int32_t  max_active;   /* resident requests, 1..4096 (0: 256)             */
     int32_t  max_slices;   /* largest n_slices accepted, 1..1024 (0: 64)      */
     int32_t  max_bypass;   /* see above (0: 8)                                */
-    int32_t  reserved;
+    int32_t  adaptive_above; /* > 0: a Parareal request submitted while this
+                              * many requests are queued or resident runs as
+                              * the plain serial solve (0: as requested)     */
 } ccfv_server_config;
 
 typedef struct ccfv_request {
     ccfv_problem problem;  /* grids: multiples of 8 cells                      */
     ccfv_ic  ic;           /* initial data at t0 when submitted without u0    */
     double   t0, t1;
-    int32_t  nt_fine, n_slices, kmax, reserved;
+    int32_t  nt_fine, n_slices, kmax, reserved;  /* n_slices 0: the plain
+                              * serial solve of nt_fine steps (bitwise
+                              * ccfv_plain_solve), PLAIN_CHUNK = 64 steps per
+                              * engine step; its report has n_slices 0     */
     double   tol;          /* as ccfv_parareal_solve                          */
 } ccfv_request;
 

@@ -395,8 +395,9 @@ class Server:
     solve requests under a device-memory budget."""
 
     def __init__(self, mem_budget: int = 0, max_active: int = 0,
-                 max_slices: int = 0, max_bypass: int = 0):
-        cfg = types.ServerConfig(mem_budget, max_active, max_slices, max_bypass, 0)
+                 max_slices: int = 0, max_bypass: int = 0, adaptive_above: int = 0):
+        cfg = types.ServerConfig(mem_budget, max_active, max_slices, max_bypass,
+                                 adaptive_above)
         h = ctypes.c_void_p()
         _check(load().ccfv_server_create(ctypes.byref(cfg), ctypes.byref(h)),
                "ccfv_server_create")

@@ -119,7 +119,7 @@ class ServerConfig(ctypes.Structure):
     """ccfv_server_config."""
     _fields_ = [("mem_budget", ctypes.c_uint64), ("max_active", ctypes.c_int32),
                 ("max_slices", ctypes.c_int32), ("max_bypass", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("adaptive_above", ctypes.c_int32)]
 
 
 class Request(ctypes.Structure):

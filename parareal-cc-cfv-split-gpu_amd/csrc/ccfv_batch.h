@@ -101,18 +101,28 @@ struct ParaJob {
     size_t sums_off = 0;  // this phase's slot sums in the shared sums buffer
     std::vector<double> slot_u2;
     ccfv_parareal_report R{};
+    // plain serial solve instead of Parareal: nt steps of dt from t0 on
+    // U[0] in place (the steps of ccfv_plain_solve), PLAIN_CHUNK per phase
+    bool plain = false;
+    int nt = 0, done = 0;
+    const double *result() const { return plain ? c->U : c->U + (size_t)ns * c->ncell; }
 };
+constexpr int PLAIN_CHUNK = 64;
 // validates the plan, sets up the tables and the report (c->U etc. must be
 // in place: ensure_para or the caller's own buffers)
 int job_setup(ParaJob &j, ccfv_ctx *c, double t0, double t1, int nt_fine, int ns,
               int kmax, double tol);
+// a plain serial solve of nt steps over [t0, t1] (c->U: one field)
+int job_setup_plain(ParaJob &j, ccfv_ctx *c, double t0, double t1, int nt);
 // Appends, for every job, its k = 0 coarse sweep if not yet issued and then
-// (if it has iterations left) one iteration.  Jobs advance in lock-step
+// (if it has iterations left) one iteration; for a plain job its next
+// PLAIN_CHUNK steps.  Jobs advance in lock-step
 // rounds; slot sums go to sums_dev at each job's sums_off (2 (ns+1) doubles),
 // assigned here; *used: the doubles of sums_dev used.
 int append_iteration(Phase &ph, std::vector<ParaJob *> &jobs, double *sums_dev,
                      size_t *used);
-// reads an iteration's slot sums (host copy of sums_dev) into the job
+// reads an iteration's slot sums (host copy of sums_dev) into the job; for
+// a plain job counts the steps of the phase
 void finish_iteration(ParaJob &j, const double *sums_host);
 
 }  // namespace rt

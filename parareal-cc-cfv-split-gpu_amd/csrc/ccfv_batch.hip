@@ -189,7 +189,7 @@ int ccfv_batch_parareal(ccfv_batch *b, const double *t0, const double *t1,
         }
         std::vector<ParaJob *> next;
         for (ParaJob *j : live) {
-            if (j->running && j->k < j->kmax) finish_iteration(*j, b->sums_host);
+            if (j->running && (j->plain || j->k < j->kmax)) finish_iteration(*j, b->sums_host);
             else j->running = false;
             if (j->running) next.push_back(j);
         }

@@ -356,6 +356,25 @@ int job_setup(ParaJob &j, ccfv_ctx *c, double t0, double t1, int nt_fine, int ns
     return CCFV_OK;
 }
 
+int job_setup_plain(ParaJob &j, ccfv_ctx *c, double t0, double t1, int nt)
+{
+    if (nt < 1) return fail(CCFV_EINVAL, "nt must be >= 1");
+    if (!(t1 > t0)) return fail(CCFV_EINVAL, "need t1 > t0");
+    j = ParaJob();
+    j.c = c;
+    j.plain = true;
+    j.nt = nt;
+    j.t0 = t0;
+    j.dt = (t1 - t0) / nt;  // as ccfv_plain_solve
+    int rc;
+    if ((rc = get_table(c, DIR_X, j.dt, &j.fx)) || (rc = get_table(c, DIR_Y, j.dt, &j.fy)))
+        return rc;
+    memset(&j.R, 0, sizeof(j.R));
+    j.started = true;
+    j.running = true;
+    return CCFV_OK;
+}
+
 namespace {
 
 // the steps of a job (the schedule of ccfv_parareal_solve)
@@ -392,8 +411,20 @@ int append_iteration(Phase &ph, std::vector<ParaJob *> &jobs, double *sums_dev,
                      size_t *used_out)
 {
     int rc;
-    // k = 0 coarse sweep: U[n+1] = G(U[n]), G cached (T2S4_Parareal.cu:1264-1280)
+    // plain jobs: their next steps, tn = t0 + n dt (T2S4_Parareal.cu:1234-1236)
     std::vector<StepRun> runs;
+    for (ParaJob *p : jobs) {
+        if (!p->plain || !p->running) continue;
+        const int n0 = p->done, cnt = std::min(PLAIN_CHUNK, p->nt - p->done);
+        runs.push_back(StepRun{p->c, cnt, 0, [p, n0](int i) {
+                                   const int n = n0 + i;
+                                   return StepDesc{p->c->U, p->c->U, 1, p->t0 + n * p->dt, 0.0, 0,
+                                                   0.0, p->dt};
+                               }});
+    }
+    if ((rc = emit_all(ph, runs))) return rc;
+    runs.clear();
+    // k = 0 coarse sweep: U[n+1] = G(U[n]), G cached (T2S4_Parareal.cu:1264-1280)
     for (ParaJob *p : jobs)
         if (!p->started)
             runs.push_back(StepRun{p->c, p->ns, 0, [p](int n) { return job_init_step(*p, n); }});
@@ -406,7 +437,7 @@ int append_iteration(Phase &ph, std::vector<ParaJob *> &jobs, double *sums_dev,
     // iterating jobs this phase
     std::vector<ParaJob *> it;
     for (ParaJob *p : jobs)
-        if (p->running && p->k < p->kmax) it.push_back(p);
+        if (!p->plain && p->running && p->k < p->kmax) it.push_back(p);
     // fine propagators of slices k..ns-1, all slices of a job as one batch
     // of fields (T2S4_Parareal.cu:1319-1324)
     runs.clear();
@@ -450,6 +481,15 @@ int append_iteration(Phase &ph, std::vector<ParaJob *> &jobs, double *sums_dev,
 
 void finish_iteration(ParaJob &p, const double *sums_host)
 {
+    if (p.plain) {
+        p.done = std::min(p.nt, p.done + PLAIN_CHUNK);
+        p.R.fine_steps_run = p.done;
+        if (p.done >= p.nt) {
+            p.R.converged = 1;  // the serial fine solution itself
+            p.running = false;
+        }
+        return;
+    }
     const double *h = sums_host + p.sums_off;
     const int k = p.k;
     double num = 0.0, den = 0.0;

@@ -29,6 +29,7 @@ namespace {
 struct Request {
     int64_t id = 0;
     ccfv_request r{};
+    bool plain = false;      // run as the plain serial solve
     std::vector<double> u0;  // empty: initial data from r.ic
     size_t bytes = 0, off = 0;
     int slot = -1;
@@ -63,8 +64,12 @@ struct ccfv_server {
 
 namespace {
 
-int request_footprint(const ccfv_request *r, size_t *bytes)
+int request_footprint(const ccfv_request *r, bool plain, size_t *bytes)
 {
+    if (plain) {  // the state field alone
+        *bytes = sched::round_up(sizeof(double) * (size_t)r->problem.nx * r->problem.ny, 256);
+        return CCFV_OK;
+    }
     // coarse-chain blocks: one per pair of x lines (the one-field x shape)
     const int nb = r->problem.ny / 2;
     *bytes = sched::request_bytes(r->problem.nx, r->problem.ny, r->n_slices, nb);
@@ -77,9 +82,10 @@ int validate_request(const ccfv_server *s, const ccfv_request *r)
     if (rc) return rc;
     if (r->problem.nx % 8 || r->problem.ny % 8)
         return fail(CCFV_EINVAL, "served grids must be multiples of 8 cells");
-    if (r->n_slices < 1 || (s && r->n_slices > s->cfg.max_slices))
+    if (r->n_slices < 0 || (s && r->n_slices > s->cfg.max_slices))
         return fail(CCFV_EINVAL, "n_slices out of range for this engine");
-    if (r->nt_fine < r->n_slices || r->nt_fine % r->n_slices)
+    if (r->n_slices == 0 ? r->nt_fine < 1
+                         : (r->nt_fine < r->n_slices || r->nt_fine % r->n_slices))
         return fail(CCFV_EINVAL, "nt_fine must be a positive multiple of n_slices");
     if (r->kmax < 0 || r->kmax > CCFV_MAX_ITERS) return fail(CCFV_EINVAL, "kmax must be in [0, 64]");
     if (!(r->t1 > r->t0)) return fail(CCFV_EINVAL, "need t1 > t0");
@@ -105,6 +111,16 @@ int place(ccfv_server *s, Request &q)
         p += sched::round_up(b, 256);
         return (double *)r;
     };
+    c.d_flag = s->d_flags + q.slot;
+    c.h_flag = s->h_flags + q.slot;
+    int rc;
+    if (q.plain) {
+        c.U = take(f);
+        c.state = nullptr;
+        if ((size_t)(p - (s->arena + q.off)) > q.bytes)
+            return fail(CCFV_EDEVICE, "request layout exceeds its arena block");
+        if ((rc = job_setup_plain(q.job, &c, q.r.t0, q.r.t1, q.r.nt_fine))) return rc;
+    } else {
     c.U = take(f * (ns + 1));
     c.G = take(f * ns);
     c.F = take(f * ns);
@@ -113,12 +129,11 @@ int place(ccfv_server *s, Request &q)
     c.partial = take(sizeof(double) * 2 * (size_t)c.chain_blocks * (ns + 1));
     c.para_ns = ns;
     c.state = nullptr;
-    c.d_flag = s->d_flags + q.slot;
-    c.h_flag = s->h_flags + q.slot;
     if ((size_t)(p - (s->arena + q.off)) > q.bytes)
         return fail(CCFV_EDEVICE, "request layout exceeds its arena block");
-    int rc = job_setup(q.job, &c, q.r.t0, q.r.t1, q.r.nt_fine, ns, q.r.kmax, q.r.tol);
-    if (rc) return rc;
+    if ((rc = job_setup(q.job, &c, q.r.t0, q.r.t1, q.r.nt_fine, ns, q.r.kmax, q.r.tol)))
+        return rc;
+    }
     HIPCHK(hipMemsetAsync(c.d_flag, 0, sizeof(int), s->run.stream));
     if (!q.u0.empty()) {
         HIPCHK(hipMemcpyAsync(c.U, q.u0.data(), f, hipMemcpyHostToDevice, s->run.stream));
@@ -143,8 +158,7 @@ int retire(ccfv_server *s, Request &q)
     const size_t f = sizeof(double) * q.ctx.ncell;
     q.result.resize(q.ctx.ncell);
     if (q.status == CCFV_OK || q.status == CCFV_ENOTCONV)
-        HIPCHK(hipMemcpy(q.result.data(), q.ctx.U + (size_t)q.r.n_slices * q.ctx.ncell, f,
-                         hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(q.result.data(), q.job.result(), f, hipMemcpyDeviceToHost));
     release_tables(q.ctx);
     s->alloc.release(q.off, q.bytes);
     s->free_slots.push_back(q.slot);
@@ -166,7 +180,7 @@ int ccfv_request_bytes(const ccfv_request *r, uint64_t *bytes)
     int rc = validate_request(nullptr, r);
     if (rc) return rc;
     size_t b = 0;
-    request_footprint(r, &b);
+    request_footprint(r, r->n_slices == 0, &b);
     *bytes = b;
     return CCFV_OK;
 }
@@ -239,14 +253,21 @@ int ccfv_server_submit(ccfv_server *s, const ccfv_request *r, const double *u0, 
     *id = -1;
     int rc = validate_request(s, r);
     if (rc) return rc;
+    // adaptive policy: a Parareal request that arrives while the engine is
+    // busy runs as the plain serial solve (the reference of its accuracy,
+    // a sixth or less of the device work)
+    const bool busy = s->cfg.adaptive_above > 0 &&
+                      s->queue.size() + s->active.size() >= (size_t)s->cfg.adaptive_above;
+    const bool plain = r->n_slices == 0 || busy;
     size_t bytes = 0;
-    request_footprint(r, &bytes);
+    request_footprint(r, plain, &bytes);
     if (sched::round_up(bytes, 256) > s->alloc.size())
         return fail(CCFV_EBUSY, "request needs " + std::to_string(bytes) +
                                     " device bytes, more than the engine's budget");
     auto q = std::make_unique<Request>();
     q->id = s->next_id++;
     q->r = *r;
+    q->plain = plain;
     q->bytes = bytes;
     if (u0) q->u0.assign(u0, u0 + (size_t)r->problem.nx * r->problem.ny);
     s->queue.push(q->id, bytes);
@@ -299,7 +320,7 @@ int ccfv_server_step(ccfv_server *s, ccfv_server_status *st)
         std::vector<Request *> still;
         for (Request *q : s->active) {
             ParaJob &j = q->job;
-            if (j.running && j.k < j.kmax) finish_iteration(j, s->sums_host);
+            if (j.running && (j.plain || j.k < j.kmax)) finish_iteration(j, s->sums_host);
             else j.running = false;
             if (s->h_flags[q->slot]) {
                 q->status = fail(CCFV_ECFL, "characteristic feet crossed: the time step is "
@@ -312,7 +333,8 @@ int ccfv_server_step(ccfv_server *s, ccfv_server_status *st)
                 still.push_back(q);
                 continue;
             }
-            if (q->status == CCFV_OK && j.tol > 0 && !j.R.converged) q->status = CCFV_ENOTCONV;
+            if (q->status == CCFV_OK && !j.plain && j.tol > 0 && !j.R.converged)
+                q->status = CCFV_ENOTCONV;
             if ((rc = retire(s, *q))) return rc;
         }
         s->active.swap(still);

@@ -161,3 +161,47 @@ def test_small_plans_and_grids(gpu):
             assert np.array_equal(c.get_state(), u)
             assert rs.iterations == rep.iterations and rs.converged == rep.converged
         assert st in (T.CCFV_OK, T.CCFV_ENOTCONV)
+
+
+def test_plain_requests_are_the_serial_solve(gpu):
+    """n_slices = 0: the plain serial solve, advanced 64 steps per engine
+    step beside Parareal requests, bitwise ccfv_plain_solve."""
+    mix = small_mix()
+    plain = [workloads.Solve(T.rotation_problem(128), T.rotation_ic(), 0.0, 0.3, 150, 0),
+             workloads.Solve(T.swirl_problem(48, T=1.0), T.cosine_bell_ic(), 0.0, 0.5, 70, 0)]
+    with ccfv.Server(mem_budget=1 << 30, max_active=16) as srv:
+        ids = [srv.submit(s.request(3, 0.0)) for s in mix[:3]]
+        pids = [srv.submit(s.request(0, 0.0)) for s in plain]
+        res = drain(srv)
+    for s, rid in zip(plain, pids):
+        u, rep, st = res[rid]
+        assert st == T.CCFV_OK and rep.n_slices == 0 and rep.converged == 1
+        assert rep.fine_steps_run == s.nt_fine
+        assert np.array_equal(u, solo_plain_state(gpu, s))
+    for s, rid in zip(mix[:3], ids):
+        assert np.array_equal(res[rid][0], alone(s, 3, 0.0)[0])
+
+
+def solo_plain_state(gpu, s):
+    with gpu.Context(s.problem) as c:
+        c.fill_exact(s.ic, s.t0)
+        c.plain_solve(s.t0, s.t1, s.nt_fine)
+        return c.get_state()
+
+
+def test_adaptive_policy_runs_busy_arrivals_serially(gpu):
+    """adaptive_above = 2: the first two requests run as Parareal; the ones
+    submitted while two are waiting run as the plain serial solve, which is
+    their serial fine solution exactly."""
+    mix = small_mix()[:5]
+    with ccfv.Server(mem_budget=1 << 30, max_active=16, adaptive_above=2) as srv:
+        ids = [srv.submit(s.request(8, 1e-9)) for s in mix]
+        res = drain(srv)
+    for i, (s, rid) in enumerate(zip(mix, ids)):
+        u, rep, st = res[rid]
+        if i < 2:
+            assert rep.n_slices == s.n_slices
+            assert np.array_equal(u, alone(s, 8, 1e-9)[0])
+        else:
+            assert rep.n_slices == 0 and st == T.CCFV_OK
+            assert np.array_equal(u, solo_plain_state(gpu, s))
