@@ -204,8 +204,11 @@ int ccfv_parareal_solve(ccfv_ctx *ctx, double t0, double t1, int32_t nt_fine,
  * every slice serially, :1302-1330):
  *   CCFV_OPT_PIPELINE   1 (default): the fine propagators of iteration k+1
  *                       run on a second stream and overlap the coarse chain
- *                       of iteration k group by group; 0: one stream, fine
- *                       batch then chain.  Both give identical results.
+ *                       of iteration k group by group; 2: in addition the
+ *                       coarse chains of consecutive iterations run on two
+ *                       streams and overlap (faster when the chain dominates:
+ *                       many slices of few fine steps); 0: one stream, fine
+ *                       batch then chain.  All give identical results.
  *   CCFV_OPT_PIPE_GROUP slices per group of the pipelined schedule (32).
  *   CCFV_OPT_GRAPHS     1: each pipelined phase (coarse chain of iteration k
  *                       with the fine propagators of k+1) is captured once

@@ -15,6 +15,7 @@ struct ccfv_ctx {
     int device = 0;
     hipStream_t stream = nullptr;   // the context's stream (coarse chain)
     hipStream_t stream2 = nullptr;  // fine propagators when pipelined
+    hipStream_t stream3 = nullptr;  // odd iterations' coarse chain (overlapped)
     hipStream_t cur = nullptr;      // stream the next sweep launches go to
     size_t ncell = 0;
     double *state = nullptr;
@@ -27,11 +28,14 @@ struct ccfv_ctx {
     int para_ns = 0;
     double *U = nullptr, *G = nullptr, *F = nullptr, *Wk = nullptr;
     double *partial = nullptr, *slot_sums = nullptr, *h_sums = nullptr;
+    // overlapped schedule: the odd iterations' work field and partials, and
+    // a copy of U[ns] per iteration parity
+    double *Wk2 = nullptr, *partial2 = nullptr, *res2 = nullptr;
     int chain_blocks = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // pipelined Parareal: fine propagators of iteration k+1 overlap the
     // coarse chain of iteration k, group by group
-    bool pipeline = true;
+    int pipeline = 1;  // 0 one stream, 1 pipelined, 2 overlapped chains
     int pipe_group = 32;
     // captured Parareal phases, keyed by plan; cleared when buffers or
     // launch shapes change

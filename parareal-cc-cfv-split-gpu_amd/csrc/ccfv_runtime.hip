@@ -320,6 +320,7 @@ hipError_t timed_sweep(ccfv_ctx *c, int kind, double bytes, Dir dir, bool post,
     if (c->recs.size() > 4096) {
         hipError_t e = hipStreamSynchronize(c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream2);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream3);
         if (e != hipSuccess) return e;
         drain_records(c);
     }
@@ -372,6 +373,7 @@ hipError_t timed_sweep_fused(ccfv_ctx *c, int kind, double bytes, bool post, Epi
     if (c->recs.size() > 4096) {
         hipError_t e = hipStreamSynchronize(c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream2);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream3);
         if (e != hipSuccess) return e;
         drain_records(c);
     }
@@ -531,6 +533,10 @@ void free_para(ccfv_ctx *c)
     (void)hipFree(c->G);
     (void)hipFree(c->F);
     (void)hipFree(c->Wk);
+    (void)hipFree(c->Wk2);
+    (void)hipFree(c->partial2);
+    (void)hipFree(c->res2);
+    c->Wk2 = c->partial2 = c->res2 = nullptr;
     (void)hipFree(c->partial);
     (void)hipFree(c->slot_sums);
     if (c->h_sums) (void)hipHostFree(c->h_sums);
@@ -550,8 +556,12 @@ int ensure_para(ccfv_ctx *c, int ns)
         (e = hipMalloc(&c->G, fb * ns)) != hipSuccess ||
         (e = hipMalloc(&c->F, fb * ns)) != hipSuccess ||
         (e = hipMalloc(&c->Wk, fb)) != hipSuccess ||
+        (e = hipMalloc(&c->Wk2, fb)) != hipSuccess ||
+        (e = hipMalloc(&c->res2, 2 * fb)) != hipSuccess ||
         (e = hipMalloc(&c->partial, sizeof(double) * 2 * c->chain_blocks *
                                         (size_t)(ns + 1))) != hipSuccess ||
+        (e = hipMalloc(&c->partial2, sizeof(double) * 2 * c->chain_blocks *
+                                         (size_t)(ns + 1))) != hipSuccess ||
         (e = hipMalloc(&c->slot_sums, sizeof(double) * 2 * (ns + 1) *
                                           CCFV_MAX_ITERS)) != hipSuccess ||
         (e = hipHostMalloc(&c->h_sums, sizeof(double) * 2 * (ns + 1) *
@@ -654,6 +664,8 @@ int ccfv_create(const ccfv_problem *p, ccfv_ctx **out)
             hipSuccess ||
         (e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking)) !=
             hipSuccess ||
+        (e = hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking)) !=
+            hipSuccess ||
         (e = hipMalloc(&c->state, sizeof(double) * c->ncell)) != hipSuccess ||
         (e = hipMalloc(&c->d_flag, sizeof(int))) != hipSuccess ||
         (e = hipHostMalloc(&c->h_flag, sizeof(int))) != hipSuccess ||
@@ -665,7 +677,7 @@ int ccfv_create(const ccfv_problem *p, ccfv_ctx **out)
     }
     *c->h_flag = 0;
     c->cur = c->stream;
-    if (const char *s = getenv("CCFV_PIPE")) c->pipeline = atoi(s) != 0;
+    if (const char *s = getenv("CCFV_PIPE")) c->pipeline = std::min(2, std::max(0, atoi(s)));
     if (const char *s = getenv("CCFV_PIPE_GROUP")) c->pipe_group = std::max(1, atoi(s));
     if (const char *s = getenv("CCFV_GRAPHS")) c->graphs = atoi(s) != 0;
     if (const char *s = getenv("CCFV_PLAIN_GRAPHS")) c->plain_graphs = atoi(s) != 0;
@@ -685,6 +697,7 @@ void ccfv_destroy(ccfv_ctx *c)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    if (c->stream3) (void)hipStreamSynchronize(c->stream3);
     free_para(c);
     drain_records(c);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -699,6 +712,7 @@ void ccfv_destroy(ccfv_ctx *c)
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
+    if (c->stream3) (void)hipStreamDestroy(c->stream3);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -900,6 +914,7 @@ int ccfv_set_tiles(ccfv_ctx *c, const int32_t t[8])
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipStreamSynchronize(c->stream2));
+    HIPCHK(hipStreamSynchronize(c->stream3));
     clear_graphs(c);
     c->tiles_version++;
     c->tile[DIR_X][0] = v[0];
@@ -996,6 +1011,7 @@ int ccfv_profile_enable(ccfv_ctx *c, int32_t enable)
     if (!c) return fail(CCFV_EINVAL, "NULL context");
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipStreamSynchronize(c->stream2));
+    HIPCHK(hipStreamSynchronize(c->stream3));
     drain_records(c);
     memset(c->stats, 0, sizeof(c->stats));
     for (int k = 0; k < CCFV_KSTAT_COUNT; k++) c->stats[k].kind = k;
@@ -1008,6 +1024,7 @@ int ccfv_profile_read(ccfv_ctx *c, ccfv_kernel_stat *out)
     if (!c || !out) return fail(CCFV_EINVAL, "NULL argument");
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipStreamSynchronize(c->stream2));
+    HIPCHK(hipStreamSynchronize(c->stream3));
     drain_records(c);
     for (int k = 0; k < CCFV_KSTAT_COUNT; k++) {
         out[k] = c->stats[k];
@@ -1336,6 +1353,194 @@ int parareal_pipelined(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R,
     return CCFV_OK;
 }
 
+// Overlapped Parareal (default when not capturing graphs).  As the
+// pipelined schedule, the fine propagators of iteration k+1 run on a second
+// stream group by group once the chain of iteration k has passed the group;
+// in addition the coarse chains of consecutive iterations run on two streams
+// (even k on the context's stream, odd k on a third) with their own work
+// field and increment partials, so the chain of iteration k+1 starts as soon
+// as its first group's fine propagators are done instead of after the whole
+// chain of iteration k.  Events per iteration parity and group order every
+// read-after-write and write-after-read on U, G and F:
+//   pass[p][g]  chain(k-1) (or the k=0 coarse sweep) has written U^k of
+//               group g and read its F and G  (p = k & 1)
+//   read[p][g]  fine(k) has read U^k of group g
+//   done[p][g]  fine(k) has written F of group g
+// chain(k+1) step n also overwrites U[n+1] and G[n], which chain(k) reads
+// at steps n+1 and n; the waits on done[][g] and read[][g+1] order it after
+// chain(k) has passed group g+1.  The chain of iteration k+1 is issued
+// before k's increment is known; each chain copies its U[ns] to a copy of
+// its parity, and the accepted iteration's copy becomes U[ns] at the end.
+int parareal_overlapped(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R,
+                        std::vector<double> &slot_u2, bool &converged, int &kdone)
+{
+    const size_t N = c->ncell;
+    const int ns = pl.ns, J = pl.J, nb = c->chain_blocks;
+    const int gs = std::min(c->pipe_group, ns);
+    const int G = (ns + gs - 1) / gs;
+    auto g0 = [&](int g) { return g * gs; };
+    auto g1 = [&](int g) { return std::min(ns, (g + 1) * gs); };
+    auto grp = [&](int n) { return n / gs; };
+    // events: [par][kind][g] (kind 0 pass, 1 read, 2 done) + end[par] + join
+    auto ev = [&](int par, int kind, int g) { return c->sync_ev[(size_t)((par * 3 + kind) * G + g)]; };
+    for (int i = 0; i < 6 * G + 3; i++)
+        if (!sync_event(c, (size_t)i)) return fail(CCFV_EDEVICE, "event creation failed");
+    hipEvent_t end_ev[2] = {c->sync_ev[(size_t)6 * G], c->sync_ev[(size_t)6 * G + 1]};
+    hipEvent_t join = c->sync_ev[(size_t)6 * G + 2];
+    hipStream_t sCh[2] = {c->stream, c->stream3}, sF = c->stream2;
+    double *wk[2] = {c->Wk, c->Wk2}, *part[2] = {c->partial, c->partial2};
+    double *res[2] = {c->res2, c->res2 + N};
+    int rc;
+
+    auto enqueue_fine = [&](int k) -> int {
+        const int par = k & 1;
+        c->cur = sF;
+        for (int g = 0; g < G; g++) {
+            const int lo = std::max(g0(g), k);
+            if (lo >= g1(g)) continue;
+            HIPCHK(hipStreamWaitEvent(sF, ev(par, 0, g), 0));
+            int r2 = run_steps(
+                c, J, [&](int j) { return fine_step(c, pl, lo, g1(g) - lo, j); }, {},
+                [&](int j) -> int {
+                    if (j == 0) HIPCHK(hipEventRecord(ev(par, 1, g), sF));
+                    return CCFV_OK;
+                });
+            if (r2) return r2;
+            HIPCHK(hipEventRecord(ev(par, 2, g), sF));
+        }
+        return CCFV_OK;
+    };
+    auto fine_count = [&](int k) {
+        double t = 0;
+        for (int g = 0; g < G; g++) {
+            const int lo = std::max(g0(g), k);
+            if (lo < g1(g)) t += (double)(g1(g) - lo) * J;
+        }
+        return t;
+    };
+    // chain(k) on sCh[k & 1] with that parity's work field and partials
+    auto enqueue_chain = [&](int k) -> int {
+        const int par = k & 1, npar = (k + 1) & 1;
+        const hipStream_t sC = sCh[par];
+        c->cur = sC;
+        // slot k+1 needs F[k]; it overwrites U[k+1], which fine(k) of the
+        // next group reads
+        HIPCHK(hipStreamWaitEvent(sC, ev(par, 2, grp(k)), 0));
+        if (k + 1 < ns) HIPCHK(hipStreamWaitEvent(sC, ev(par, 1, grp(k + 1)), 0));
+        HIPCHK(launch_correction(c->G + k * N, c->G + k * N, c->F + k * N,
+                                 c->U + (k + 1) * N, N,
+                                 part[par] + 2 * (size_t)nb * (k + 1), nb, sC));
+        auto step = [&](int i) {
+            const int n = k + 1 + i;
+            return StepDesc{c->U + n * N, wk[par], 1, pl.t0, pl.DT, n, 0.0, pl.DT,
+                            EPI_CORRECT, c->G + n * N, c->F + n * N, c->U + (n + 1) * N,
+                            part[par] + 2 * (size_t)nb * (n + 1)};
+        };
+        int r2 = run_steps(
+            c, ns - k - 1, step,
+            [&](int i) -> int {
+                const int n = k + 1 + i, g = grp(n);
+                if (n == k + 1 || n == g0(g)) {
+                    HIPCHK(hipStreamWaitEvent(sC, ev(par, 2, g), 0));
+                    if (g + 1 < G) HIPCHK(hipStreamWaitEvent(sC, ev(par, 1, g + 1), 0));
+                }
+                return CCFV_OK;
+            },
+            [&](int i) -> int {
+                const int n = k + 1 + i, g = grp(n);
+                // the last group's pass follows the copy of U[ns] below
+                if (n == g1(g) - 1 && g + 1 < G) HIPCHK(hipEventRecord(ev(npar, 0, g), sC));
+                return CCFV_OK;
+            });
+        if (r2) return r2;
+        HIPCHK(launch_copy(c->U + (size_t)ns * N, res[par], N, sC));
+        if (k + 1 < ns) HIPCHK(hipEventRecord(ev(npar, 0, G - 1), sC));
+        double *sums = c->slot_sums + (size_t)k * 2 * (ns + 1);
+        HIPCHK(launch_reduce_segments(part[par], nb, k + 1, ns - k, sums, sC));
+        HIPCHK(hipMemcpyAsync(c->h_sums + (size_t)k * 2 * (ns + 1), sums,
+                              sizeof(double) * 2 * (ns + 1), hipMemcpyDeviceToHost, sC));
+        HIPCHK(hipEventRecord(end_ev[par], sC));
+        return CCFV_OK;
+    };
+    // one finished iteration's increment and the stop decision
+    auto check = [&](int k, bool &stop) -> int {
+        HIPCHK(hipEventSynchronize(end_ev[k & 1]));
+        const double inc = increment_of(c, ns, k, slot_u2);
+        R.increment[k] = inc;
+        stop = false;
+        if (!std::isfinite(inc)) return fail(CCFV_ENONFINITE, "Parareal iterate is not finite");
+        if (pl.tol > 0 && inc <= pl.tol) {
+            converged = true;
+            stop = true;
+        }
+        return CCFV_OK;
+    };
+
+    // everything queued on the context's stream before (the initial copy)
+    // comes first on every stream
+    HIPCHK(hipEventRecord(join, c->stream));
+    HIPCHK(hipStreamWaitEvent(sF, join, 0));
+    HIPCHK(hipStreamWaitEvent(c->stream3, join, 0));
+    // k = 0 coarse sweep on the even chain stream
+    c->cur = sCh[0];
+    rc = run_steps(
+        c, ns, [&](int n) { return init_step(c, pl, n); }, {},
+        [&](int n) -> int {
+            if (n == g1(grp(n)) - 1) HIPCHK(hipEventRecord(ev(0, 0, grp(n)), sCh[0]));
+            return CCFV_OK;
+        });
+    if (rc) return rc;
+    R.coarse_steps_run = ns;
+    if (pl.kmax > 0) {
+        if ((rc = enqueue_fine(0))) return rc;
+        R.fine_steps_run += fine_count(0);
+    }
+    int accepted = -1;  // last iteration whose increment has been read
+    bool stop = false;
+    int k;
+    for (k = 0; k < pl.kmax && !stop; k++) {
+        if ((rc = enqueue_chain(k))) return rc;
+        R.coarse_steps_run += ns - k - 1;
+        if (k + 1 < pl.kmax) {
+            if ((rc = enqueue_fine(k + 1))) return rc;
+            R.fine_steps_run += fine_count(k + 1);
+        }
+        // read iteration k-1 while chain(k) runs
+        if (pl.tol > 0 && k >= 1) {
+            if ((rc = check(k - 1, stop))) {
+                kdone = k;
+                return rc;
+            }
+            accepted = k - 1;
+        }
+    }
+    int last = stop ? accepted : k - 1;  // the iteration whose U[ns] is the result
+    if (!stop && last >= 0 && pl.tol > 0) {
+        if ((rc = check(last, stop))) {
+            kdone = last + 1;
+            return rc;
+        }
+    }
+    kdone = last + 1;
+    // join every stream into the context's stream (speculative chains too)
+    for (hipStream_t s : {sF, c->stream3}) {
+        HIPCHK(hipEventRecord(join, s));
+        HIPCHK(hipStreamWaitEvent(c->stream, join, 0));
+    }
+    if (last >= 0)
+        HIPCHK(launch_copy(res[last & 1], c->U + (size_t)ns * N, N, c->stream));
+    c->cur = c->stream;
+    if (pl.tol <= 0) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < kdone; i++) {
+            R.increment[i] = increment_of(c, ns, i, slot_u2);
+            if (!std::isfinite(R.increment[i]))
+                return fail(CCFV_ENONFINITE, "Parareal iterate is not finite");
+        }
+    }
+    return CCFV_OK;
+}
+
 }  // namespace rt
 }  // namespace ccfv
 
@@ -1378,8 +1583,10 @@ int ccfv_parareal_solve(ccfv_ctx *c, double t0, double t1, int32_t nt_fine,
 
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     HIPCHK(launch_copy(c->state, c->U, N, c->stream));
-    rc = c->pipeline ? parareal_pipelined(c, pl, R, slot_u2, converged, kdone)
-                     : parareal_sequential(c, pl, R, slot_u2, converged, kdone);
+    if (!c->pipeline) rc = parareal_sequential(c, pl, R, slot_u2, converged, kdone);
+    else if (c->pipeline == 2 && !(c->graphs && !c->prof))
+        rc = parareal_overlapped(c, pl, R, slot_u2, converged, kdone);
+    else rc = parareal_pipelined(c, pl, R, slot_u2, converged, kdone);
     c->cur = c->stream;
     if (rc == CCFV_OK && kdone == ns) converged = true;  // every slice exact
     R.iterations = kdone;
@@ -1387,6 +1594,7 @@ int ccfv_parareal_solve(ccfv_ctx *c, double t0, double t1, int32_t nt_fine,
     if (rc) {
         (void)hipStreamSynchronize(c->stream);
         (void)hipStreamSynchronize(c->stream2);
+        (void)hipStreamSynchronize(c->stream3);
         if (rep) *rep = R;
         return rc;
     }
@@ -1408,9 +1616,11 @@ int ccfv_set_option(ccfv_ctx *c, int32_t key, int64_t value)
     if (!c) return fail(CCFV_EINVAL, "NULL context");
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipStreamSynchronize(c->stream2));
+    HIPCHK(hipStreamSynchronize(c->stream3));
     switch (key) {
     case CCFV_OPT_PIPELINE:
-        c->pipeline = value != 0;
+        if (value < 0 || value > 2) return fail(CCFV_EINVAL, "pipeline must be 0, 1 or 2");
+        c->pipeline = (int)value;
         return CCFV_OK;
     case CCFV_OPT_PIPE_GROUP:
         if (value < 1 || value > 4096) return fail(CCFV_EINVAL, "group size out of range");
@@ -1436,7 +1646,7 @@ int ccfv_get_option(ccfv_ctx *c, int32_t key, int64_t *value)
 {
     if (!c || !value) return fail(CCFV_EINVAL, "NULL argument");
     switch (key) {
-    case CCFV_OPT_PIPELINE: *value = c->pipeline ? 1 : 0; return CCFV_OK;
+    case CCFV_OPT_PIPELINE: *value = c->pipeline; return CCFV_OK;
     case CCFV_OPT_PIPE_GROUP: *value = c->pipe_group; return CCFV_OK;
     case CCFV_OPT_GRAPHS: *value = c->graphs ? 1 : 0; return CCFV_OK;
     case CCFV_OPT_PLAIN_GRAPHS: *value = c->plain_graphs ? 1 : 0; return CCFV_OK;

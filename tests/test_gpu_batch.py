@@ -77,7 +77,9 @@ def test_batch_parareal_matches_solo(gpu, kmax, tol):
         J = nt[i] // ns[i]
         its = reps[i].iterations
         assert reps[i].fine_steps_run == sum((ns[i] - k) * J for k in range(its)), i
-        assert reps[i].coarse_steps_run == rr.coarse_steps_run, i
+        assert reps[i].coarse_steps_run == ns[i] + sum(ns[i] - k - 1 for k in range(its)), i
+        # the single-problem schedule may have run one more speculative chain
+        assert rr.coarse_steps_run >= reps[i].coarse_steps_run, i
         np.testing.assert_allclose(reps[i].increments, rr.increments, rtol=1e-12)
 
 
@@ -165,4 +167,4 @@ def test_batch_small_plans(gpu, kmax, ns):
         assert np.array_equal(got[i], ref), i
         assert reps[i].iterations == rr.iterations == min(kmax, ns)
         assert reps[i].converged == rr.converged
-        assert reps[i].coarse_steps_run == rr.coarse_steps_run
+        assert reps[i].coarse_steps_run == ns + sum(ns - k - 1 for k in range(min(kmax, ns)))
