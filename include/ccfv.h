@@ -91,7 +91,28 @@ typedef struct ccfv_problem {
     double   vel[4];        /* velocity parameters, see CCFV_VEL_*             */
     uint32_t flags;         /* CCFV_FLAG_*                                      */
     uint32_t reserved;
+    int32_t  src_kind;      /* CCFV_SRC_* (reference: f == 0, :59-61)          */
+    uint32_t src_reserved;
+    double   src[5];        /* source parameters, see CCFV_SRC_*               */
 } ccfv_problem;
+
+/* ---- source term f(x, y, t) ---------------------------------------------
+ * Every sub-step adds theta*h*(fbar0 + f1) to its right-hand side, as the
+ * reference does (:564-613, :839-958): f1 the cell averages of f (3x3
+ * Simpson, :143-165) at the sub-step's later time, fbar0 those at its
+ * earlier time remapped over the sub-step's own feet like the solution;
+ * x sub-step 1: times tn, tn+h/2; y: tn, tn+h; x sub-step 3: tn+h for both;
+ * theta = 1/4 for x and 1/2 for y.  Over a step this adds 2*h*f for a
+ * constant f, twice the source of the equation: a solver for
+ * u_t + V.grad u = nu lap u + g takes f = g/2.
+ * NONE:     f = 0 (the reference).
+ * CONSTANT: f = src[0].
+ * SINSIN:   f = (src[1] cos(w t) + src[2] sin(w t)) S(x) S(y)
+ *             + sin(w t) (src[3] C(x) S(y) + src[4] S(x) C(y)),
+ *           w = src[0], S(z) = sin(2 pi z), C(z) = cos(2 pi z).           */
+#define CCFV_SRC_NONE     0
+#define CCFV_SRC_CONSTANT 1
+#define CCFV_SRC_SINSIN   2
 
 /* ---- initial / exact data ----------------------------------------------- */
 /* ROT_GAUSSIAN: the reference's C_Exact (:51-56) generalised to a centre of

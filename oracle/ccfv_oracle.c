@@ -277,20 +277,17 @@ void orc_cyclic(double a, double b, const double *r, int n, double *x)
 /* one line                                                                  */
 /* ------------------------------------------------------------------------ */
 /* workspace doubles for a line of n cells */
-#define LINE_WS(n) (14 * ((size_t)(n) + 4))
+#define LINE_WS(n) (15 * ((size_t)(n) + 4))
 
-static int line_sweep_ws(const double *c, const double *feet, int n, double lo,
-                         double hi, int periodic, int quirk, double eps,
-                         double nu, double h, double theta, int deriv_postremap,
-                         double *out, double *ws)
+/* PPM coefficients and the remap of one line over the faces' feet
+ * (T2S4_Parareal.cu:260-347, :349-438); cl/dl/c6 n doubles each, fm/fxi
+ * the periodic feet cells (n+1).  Returns 0 or CCFV_ECFL. */
+static int remap_line(const double *c, const double *feet, int n, double lo,
+                      double hi, int periodic, double eps, double *cl,
+                      double *dl, double *c6, int *fm, double *fxi,
+                      double *cbar)
 {
     const double d = (hi - lo) / n;
-    double *cl = ws, *dl = ws + (n + 4), *c6 = ws + 2 * (n + 4);
-    double *cbar = ws + 3 * (n + 4), *D = ws + 4 * (n + 4);
-    double *b = ws + 5 * (n + 4), *r = ws + 6 * (n + 4);
-    int *fm = (int *)(ws + 7 * (n + 4)); /* n+1 ints fit in (n+4) doubles */
-    double *fxi = ws + 8 * (n + 4);
-    double *tws = ws + 9 * (n + 4);     /* 5 blocks of solver workspace */
     double g[4] = {0, 0, 0, 0};
     if (!periodic) orc_ghosts(c, n, g);
 
@@ -362,6 +359,32 @@ static int line_sweep_ws(const double *c, const double *feet, int n, double lo,
             cbar[i] = v;
         }
     }
+    return 0;
+}
+
+static int line_sweep_ws(const double *c, const double *feet, int n, double lo,
+                         double hi, int periodic, int quirk, double eps,
+                         double nu, double h, double theta, int deriv_postremap,
+                         const double *f0, const double *f1, double *out,
+                         double *ws)
+{
+    const double d = (hi - lo) / n;
+    double *cl = ws, *dl = ws + (n + 4), *c6 = ws + 2 * (n + 4);
+    double *cbar = ws + 3 * (n + 4), *D = ws + 4 * (n + 4);
+    double *b = ws + 5 * (n + 4), *r = ws + 6 * (n + 4);
+    int *fm = (int *)(ws + 7 * (n + 4)); /* n+1 ints fit in (n+4) doubles */
+    double *fxi = ws + 8 * (n + 4);
+    double *tws = ws + 9 * (n + 4);     /* 5 blocks of solver workspace */
+    double *fbar = ws + 14 * (n + 4);
+    int rc = remap_line(c, feet, n, lo, hi, periodic, eps, cl, dl, c6, fm, fxi, cbar);
+    if (rc) return rc;
+    /* the source's earlier cell averages remapped over the same feet
+     * (T2S4_Parareal.cu:846-853, :893-900, :943-950) */
+    if (f0) {
+        rc = remap_line(f0, feet, n, lo, hi, periodic, eps, tws, tws + (n + 4),
+                        tws + 2 * (n + 4), fm, fxi, fbar);
+        if (rc) return rc;
+    }
 
     /* foot derivatives (T2S4_Parareal.cu:440-547) on the pre-remap line, or
      * on the remapped line for the reference's third sub-step (:929-932) */
@@ -387,9 +410,15 @@ static int line_sweep_ws(const double *c, const double *feet, int n, double lo,
         D[n] = D[0];
     }
 
-    /* RHS (T2S4_Parareal.cu:564-580, :596-613), f == 0 */
+    /* RHS (T2S4_Parareal.cu:564-580, :596-613) */
     const double coef = theta * nu * (h / d);
-    for (int i = 0; i < n; i++) b[i] = cbar[i] + coef * (D[i + 1] - D[i]);
+    if (f0) {
+        const double w = theta * h;
+        for (int i = 0; i < n; i++)
+            b[i] = cbar[i] + w * (fbar[i] + f1[i]) + coef * (D[i + 1] - D[i]);
+    } else {
+        for (int i = 0; i < n; i++) b[i] = cbar[i] + coef * (D[i + 1] - D[i]);
+    }
 
     /* compact operator (T2S4_Parareal.cu:616-654) */
     if (!periodic) {
@@ -424,9 +453,38 @@ int orc_line_sweep(const double *c, const double *feet, int n, double lo,
     if (n < 4) return CCFV_EINVAL;
     double *ws = (double *)malloc(sizeof(double) * LINE_WS(n));
     int rc = line_sweep_ws(c, feet, n, lo, hi, periodic, quirk, eps, nu, h,
-                           theta, deriv_postremap, out, ws);
+                           theta, deriv_postremap, NULL, NULL, out, ws);
     free(ws);
     return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* source term (T2S4_Parareal.cu:59-61, :143-165)                           */
+/* ------------------------------------------------------------------------ */
+static double src_point(const ccfv_problem *p, double x, double y, double t)
+{
+    if (p->src_kind == CCFV_SRC_CONSTANT) return p->src[0];
+    const double tp = 2.0 * M_PI, w = p->src[0];
+    const double sx = sin(tp * x), sy = sin(tp * y);
+    const double cx = cos(tp * x), cy = cos(tp * y);
+    const double ct = cos(w * t), st = sin(w * t);
+    return (p->src[1] * ct + p->src[2] * st) * sx * sy +
+           st * (p->src[3] * cx * sy + p->src[4] * sx * cy);
+}
+
+/* cell average of f by the 3x3 Simpson rule, summed as :155-160 */
+static double src_avg(const ccfv_problem *p, int i, int j, double t)
+{
+    const double dx = (p->x_hi - p->x_lo) / p->nx, dy = (p->y_hi - p->y_lo) / p->ny;
+    const double x = p->x_lo + (i + 0.5) * dx, y = p->y_lo + (j + 0.5) * dy;
+    const double xm = x - 0.5 * dx, xp = x + 0.5 * dx;
+    const double ym = y - 0.5 * dy, yp = y + 0.5 * dy;
+    return 1.0 / 36.0 *
+           (src_point(p, xm, ym, t) + src_point(p, xp, ym, t) + src_point(p, xm, yp, t) +
+            src_point(p, xp, yp, t) +
+            4.0 * (src_point(p, xm, y, t) + src_point(p, xp, y, t) + src_point(p, x, ym, t) +
+                   src_point(p, x, yp, t)) +
+            16.0 * src_point(p, x, y, t));
 }
 
 /* ------------------------------------------------------------------------ */
@@ -446,23 +504,36 @@ int orc_sweep_x(const ccfv_problem *p, const double *cin, double *cout,
     const double tau = 0.5 * h;
     const double t_lo = substep == 3 ? tn + 0.5 * h : tn;
     const double t_hi = substep == 3 ? tn + h : tn + 0.5 * h;
+    /* source times: sub-step 1 f(tn) remapped + f(tn+h/2) (:839-870);
+     * sub-step 3 f(tn+h) for both (:935-958) */
+    const int src = p->src_kind != CCFV_SRC_NONE;
+    const double tf0 = substep == 3 ? tn + h : tn;
+    const double tf1 = substep == 3 ? tn + h : tn + 0.5 * h;
     int err = 0;
 #pragma omp parallel num_threads(g_threads) reduction(| : err)
     {
         double *line = (double *)malloc(sizeof(double) * (nx + 1));
         double *res = (double *)malloc(sizeof(double) * (nx + 1));
         double *feet = (double *)malloc(sizeof(double) * (nx + 1));
+        double *f0 = (double *)malloc(sizeof(double) * (nx + 1));
+        double *f1 = (double *)malloc(sizeof(double) * (nx + 1));
         double *ws = (double *)malloc(sizeof(double) * LINE_WS(nx));
 #pragma omp for schedule(static)
         for (int j = 0; j < ny; j++) {
             for (int i = 0; i < nx; i++) line[i] = cin[(size_t)i * ny + j];
+            if (src)
+                for (int i = 0; i < nx; i++) {
+                    f0[i] = src_avg(p, i, j, tf0);
+                    f1[i] = src_avg(p, i, j, tf1);
+                }
             orc_feet_x(p, j, tau, t_hi, t_lo, feet);
             int rc = line_sweep_ws(line, feet, nx, p->x_lo, p->x_hi, periodic,
-                                   quirk, 1e-15, p->nu, h, 0.25, post, res, ws);
+                                   quirk, 1e-15, p->nu, h, 0.25, post,
+                                   src ? f0 : NULL, src ? f1 : NULL, res, ws);
             if (rc) err |= 1;
             for (int i = 0; i < nx; i++) cout[(size_t)i * ny + j] = res[i];
         }
-        free(line); free(res); free(feet); free(ws);
+        free(line); free(res); free(feet); free(f0); free(f1); free(ws);
     }
     return err ? CCFV_ECFL : 0;
 }
@@ -473,20 +544,30 @@ int orc_sweep_y(const ccfv_problem *p, const double *cin, double *cout,
     const int nx = p->nx, ny = p->ny;
     const int periodic = p->bc == CCFV_BC_PERIODIC;
     const int quirk = (p->flags & CCFV_FLAG_DERIV_GHOST_QUIRK) != 0;
+    /* source: f(tn) remapped + f(tn+h) (:873-917) */
+    const int src = p->src_kind != CCFV_SRC_NONE;
     int err = 0;
 #pragma omp parallel num_threads(g_threads) reduction(| : err)
     {
         double *feet = (double *)malloc(sizeof(double) * (ny + 1));
+        double *f0 = (double *)malloc(sizeof(double) * (ny + 1));
+        double *f1 = (double *)malloc(sizeof(double) * (ny + 1));
         double *ws = (double *)malloc(sizeof(double) * LINE_WS(ny));
 #pragma omp for schedule(static)
         for (int i = 0; i < nx; i++) {
+            if (src)
+                for (int j = 0; j < ny; j++) {
+                    f0[j] = src_avg(p, i, j, tn);
+                    f1[j] = src_avg(p, i, j, tn + h);
+                }
             orc_feet_y(p, i, h, tn + h, tn, feet);
             int rc = line_sweep_ws(cin + (size_t)i * ny, feet, ny, p->y_lo,
                                    p->y_hi, periodic, quirk, 1e-16, p->nu, h,
-                                   0.5, 0, cout + (size_t)i * ny, ws);
+                                   0.5, 0, src ? f0 : NULL, src ? f1 : NULL,
+                                   cout + (size_t)i * ny, ws);
             if (rc) err |= 1;
         }
-        free(feet); free(ws);
+        free(feet); free(f0); free(f1); free(ws);
     }
     return err ? CCFV_ECFL : 0;
 }

@@ -35,6 +35,10 @@ FLAG_STEP3_POSTREMAP = 0x1
 FLAG_DERIV_GHOST_QUIRK = 0x2
 FLAGS_REFERENCE = FLAG_STEP3_POSTREMAP | FLAG_DERIV_GHOST_QUIRK
 
+SRC_NONE = 0
+SRC_CONSTANT = 1
+SRC_SINSIN = 2
+
 IC_ROT_GAUSSIAN = 0
 IC_GAUSSIANS = 1
 IC_SINSIN = 2
@@ -58,15 +62,20 @@ class Problem(ctypes.Structure):
         ("y_lo", ctypes.c_double), ("y_hi", ctypes.c_double),
         ("nu", ctypes.c_double), ("vel", ctypes.c_double * 4),
         ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+        ("src_kind", ctypes.c_int32), ("src_reserved", ctypes.c_uint32),
+        ("src", ctypes.c_double * 5),
     ]
 
     def copy(self, **kw) -> "Problem":
         p = Problem()
         ctypes.pointer(p)[0] = self
         for k, v in kw.items():
-            if k == "vel":
+            if k in ("vel", "src"):
+                arr = getattr(p, k)
+                for i in range(len(arr)):
+                    arr[i] = 0.0
                 for i, x in enumerate(v):
-                    p.vel[i] = x
+                    arr[i] = x
             else:
                 setattr(p, k, v)
         return p
@@ -240,3 +249,24 @@ def constant_gaussians(nx, ny, u, v, nu, bumps, flags=0):
     for b in bumps:
         params.extend(b)
     return p, make_ic(IC_GAUSSIANS, params, n=len(bumps))
+
+
+def manufactured_sinsin(n: int, *, u: float = 1.0, v: float = 0.5, nu: float = 0.01,
+                        A: float = 0.5, w: float = 2.0 * math.pi, flags: int = 0):
+    """Constant velocity (u, v), diffusion nu, and a source whose exact
+    solution is the decaying translated sin*sin plus A sin(w t) S(x) S(y)
+    (S(z) = sin(2 pi z)).  The scheme adds 2 h f per step (every sub-step
+    adds theta h (fbar0 + f1)), so f is half of the equation's source
+    g = A [w cos(w t) + 8 pi^2 nu sin(w t)] S S
+        + 2 pi A sin(w t) (u C(x) S(y) + v S(x) C(y)).
+    Returns (problem, ic, exact_extra) with exact_extra(t) the amplitude
+    A sin(w t) of the S(x) S(y) part."""
+    p = advection_sinsin_problem(n, flags=flags).copy(vel=(u, v), nu=nu)
+    tp = 2.0 * math.pi
+    p.src_kind = SRC_SINSIN
+    half = 0.5
+    for i, x in enumerate((w, half * A * w, half * A * 2.0 * tp * tp * nu,
+                           half * A * tp * u, half * A * tp * v)):
+        p.src[i] = x
+    ic = make_ic(IC_SINSIN, (u, v, nu))
+    return p, ic, (lambda t: A * math.sin(w * t))

@@ -70,7 +70,7 @@ int hip_fail(hipError_t e, const char *what);
         if (e_ != hipSuccess) return ccfv::rt::hip_fail(e_, #expr); \
     } while (0)
 
-bool tile_ok(Tile t, int n, int nl, Dir dir, bool post);
+bool tile_ok(Tile t, int n, int nl, Dir dir, bool post, bool src = false);
 double cell_width(const ccfv_problem &P, Dir dir);
 int get_table(ccfv_ctx *c, Dir dir, double h, const SolveTable **out);
 SweepArgs make_args(ccfv_ctx *c, Dir dir, int substep, const SolveTable &T,

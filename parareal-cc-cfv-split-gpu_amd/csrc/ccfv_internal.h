@@ -72,6 +72,11 @@ struct SweepArgs {
     double *out2;
     int field0_2;
     double t0_2, jdt2;
+    // source term (CCFV_SRC_*): theta*h (src_w) times the remapped earlier
+    // and the later cell averages
+    int src_kind;
+    double src[5];
+    double src_w;
 };
 
 struct Tile {
@@ -95,7 +100,7 @@ struct RaggedItem {
 hipError_t launch_sweep_generic(Dir dir, int bc, bool post, Epi epi,
                                 const SweepArgs &a, int nfields, Tile tile,
                                 hipStream_t s, bool fuse = false);
-size_t sweep_lds_bytes_generic(int n, Tile tile);
+size_t sweep_lds_bytes_generic(int n, Tile tile, bool src = false);
 // uniform-displacement sweeps (ccfv_sweep_fast.hip); periodic only
 hipError_t launch_sweep_fast(Dir dir, Epi epi, const SweepArgs &a, int nfields,
                              Tile tile, hipStream_t s, bool fuse = false);

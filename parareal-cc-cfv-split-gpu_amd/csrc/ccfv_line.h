@@ -186,6 +186,67 @@ struct Face {
     double D;         // derivative at the foot
 };
 
+// ---------------------------------------------------------------------------
+// source term f(x, y, t) (CCFV_SRC_*; T2S4_Parareal.cu:59-61, :143-165)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double src_point(const SweepArgs &a, double x, double y, double t)
+{
+#pragma clang fp contract(off)
+    if (a.src_kind == CCFV_SRC_CONSTANT) return a.src[0];
+    const double tp = 2.0 * M_PI, w = a.src[0];
+    const double sx = sin(tp * x), sy = sin(tp * y);
+    const double cx = cos(tp * x), cy = cos(tp * y);
+    const double ct = cos(w * t), st = sin(w * t);
+    return (a.src[1] * ct + a.src[2] * st) * sx * sy +
+           st * (a.src[3] * cx * sy + a.src[4] * sx * cy);
+}
+
+// cell average of f over cell q of line L by the 3x3 Simpson rule, summed
+// as T2S4_Parareal.cu:155-160 (x: along-line coordinate for DIR 0)
+template <int DIR>
+__device__ __forceinline__ double src_avg(const SweepArgs &a, int q, int L, double t)
+{
+#pragma clang fp contract(off)
+    const double s = a.lo + (q + 0.5) * a.d, o = a.olo + (L + 0.5) * a.od;
+    const double x = DIR == 0 ? s : o, y = DIR == 0 ? o : s;
+    const double dx = DIR == 0 ? a.d : a.od, dy = DIR == 0 ? a.od : a.d;
+    const double xm = x - 0.5 * dx, xp = x + 0.5 * dx;
+    const double ym = y - 0.5 * dy, yp = y + 0.5 * dy;
+    return 1.0 / 36.0 *
+           (src_point(a, xm, ym, t) + src_point(a, xp, ym, t) + src_point(a, xm, yp, t) +
+            src_point(a, xp, yp, t) +
+            4.0 * (src_point(a, xm, y, t) + src_point(a, xp, y, t) + src_point(a, x, ym, t) +
+                   src_point(a, x, yp, t)) +
+            16.0 * src_point(a, x, y, t));
+}
+
+// times of the source's earlier (remapped) and later cell averages of a
+// sub-step: x sub-step 1 [tn, tn+h/2], y [tn, tn+h], x sub-step 3 both tn+h
+// (T2S4_Parareal.cu:839-845, :887-892, :935-940)
+struct SrcTimes {
+    double t0, t1;
+};
+
+__device__ __forceinline__ SrcTimes src_times(const SweepArgs &a, int f, bool next)
+{
+#pragma clang fp contract(off)
+    const double tn = next ? a.t0_2 + (double)(a.field0_2 + f) * a.tDT + a.jdt2
+                           : a.t0 + (double)(a.field0 + f) * a.tDT + a.jdt;
+    const int sub = next ? 1 : a.substep;
+    SrcTimes t;
+    if (sub == 1) {
+        t.t0 = tn;
+        t.t1 = tn + 0.5 * a.h;
+    } else if (sub == 3) {
+        t.t0 = tn + a.h;
+        t.t1 = tn + a.h;
+    } else {
+        t.t0 = tn;
+        t.t1 = tn + a.h;
+    }
+    return t;
+}
+
 // Closed box foot derivative: zero outside the box, else clamp to
 // [lo+1e-15, hi-1e-15] (T2S4_Parareal.cu:457-490, :514-543).
 template <int BC>
@@ -311,19 +372,48 @@ __device__ __forceinline__ double remap_cell(const SweepArgs &a,
 
 // RHS b of the cells of this lane by the general per-face path; b goes to the
 // lane's own slots of `aux` (an LDS line used as an indexable register file).
+// The source's earlier cell averages remapped over the same feet, plus its
+// later ones, weighted by theta*h: the source part of the RHS of cell q.
+// fs: that line's earlier cell averages in LDS, fg its ghosts.
+template <int DIR, int BC>
+__device__ __forceinline__ double src_rhs(const SweepArgs &a, const Times &tt,
+                                          const double *fs, const double fg[4],
+                                          const LineGeo &lg, int line, int q,
+                                          const Face &gl, const Face &gr, double t1,
+                                          bool &bad)
+{
+    (void)tt;
+    (void)lg;
+    const double fbar = remap_cell<BC>(a, fs, gl, gr, bad);
+    return a.src_w * (fbar + src_avg<DIR>(a, q, line, t1));
+}
+
 template <int DIR, int BC>
 __device__ void rhs_general(const SweepArgs &a, const Times &tt, const double *cs,
                             const double g[4], double *aux, int base, int ch,
-                            const LineGeo &lg, double inv12d, bool post)
+                            const LineGeo &lg, double inv12d, bool post,
+                            const double *fs = nullptr, const double *fg = nullptr,
+                            int line = 0, double t1 = 0.0)
 {
     bool bad = false;
     Face fl = post ? eval_face<DIR, BC, true>(a, tt, cs, g, base, lg, inv12d)
                    : eval_face<DIR, BC, false>(a, tt, cs, g, base, lg, inv12d);
+    // with the third-sub-step quirk the source joins after the derivative
+    // of the remapped line (the caller adds it)
+    const bool src = fs && !post;
+    Face gl;
+    if (src) gl = eval_face<DIR, BC, true>(a, tt, fs, fg, base, lg, inv12d);
 #pragma unroll 1
     for (int k = 0; k < ch; k++) {
         Face fr = post ? eval_face<DIR, BC, true>(a, tt, cs, g, base + k + 1, lg, inv12d)
                        : eval_face<DIR, BC, false>(a, tt, cs, g, base + k + 1, lg, inv12d);
         double cbar = remap_cell<BC>(a, cs, fl, fr, bad);
+        if (src) {
+            // b = C + theta h (fbar0 + f1) + ... (T2S4_Parareal.cu:575, :607)
+            Face gr = eval_face<DIR, BC, true>(a, tt, fs, fg, base + k + 1, lg, inv12d);
+            cbar = cbar + src_rhs<DIR, BC>(a, tt, fs, fg, lg, line, base + k, gl, gr, t1, bad);
+            gl = gr;
+        }
         aux[pidx(base + k)] = post ? cbar : cbar + a.coef * (fr.D - fl.D);
         fl = fr;
     }

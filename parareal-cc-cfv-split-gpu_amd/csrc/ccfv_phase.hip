@@ -22,7 +22,8 @@ void shared_tiles(ccfv_ctx *c)
     const bool post = post_flag(c);
     const Tile x1{2, 2}, y1{2, 2}, yb{2, 1};
     Tile xb{8, 4};
-    if (!tile_ok(xb, c->P.nx, c->P.ny, DIR_X, post)) xb = Tile{2, 2};
+    if (!tile_ok(xb, c->P.nx, c->P.ny, DIR_X, post, c->P.src_kind != CCFV_SRC_NONE))
+        xb = Tile{2, 2};
     c->tile[DIR_X][0] = x1;
     c->tile[DIR_X][1] = xb;
     c->tile[DIR_Y][0] = y1;
@@ -76,7 +77,8 @@ void Phase::add_sweep_stage(std::vector<StageItem> &items)
             it.push_back(s->item);
             prefix.push_back(prefix.back() + s->item.nfields * s->item.ntiles);
             size_t b = k.fast ? sweep_lds_bytes_fast(s->item.a.n, s->tile)
-                              : sweep_lds_bytes_generic(s->item.a.n, s->tile);
+                              : sweep_lds_bytes_generic(s->item.a.n, s->tile,
+                                                        s->item.a.src_kind != CCFV_SRC_NONE);
             lds = std::max(lds, b);
         }
         Launch L;

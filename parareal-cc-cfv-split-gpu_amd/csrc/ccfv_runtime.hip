@@ -53,19 +53,19 @@ Tile env_tile(const char *name, Tile dflt)
     return t;
 }
 
-bool tile_ok(Tile t, int n, int nl, Dir dir, bool post)
+bool tile_ok(Tile t, int n, int nl, Dir dir, bool post, bool src)
 {
     (void)post;
     // NW <= 4: the sweep kernels are compiled for at most 256 work-items
     if (t.w <= 0 || t.nw <= 0 || t.nw > 4 || t.w % t.nw || nl % t.w) return false;
     if (t.w & (t.w - 1)) return false;  // power of two: tiles split by shifts
     if (dir == DIR_X && (t.w & 1)) return false;
-    return std::max(sweep_lds_bytes_generic(n, t), sweep_lds_bytes_fast(n, t)) <=
+    return std::max(sweep_lds_bytes_generic(n, t, src), sweep_lds_bytes_fast(n, t)) <=
            160 * 1024;
 }
 
 // default tiles (measured on MI355X at 512^2); ccfv_autotune refines them
-Tile pick_tile(Dir dir, int n, int nl, bool batch, bool post)
+Tile pick_tile(Dir dir, int n, int nl, bool batch, bool post, bool src)
 {
     static const Tile xb[] = {{8, 4}, {16, 4}, {4, 4}, {4, 2}, {2, 2}, {2, 1}};
     static const Tile x1[] = {{2, 2}, {4, 4}, {4, 2}, {2, 1}, {8, 4}};
@@ -74,7 +74,7 @@ Tile pick_tile(Dir dir, int n, int nl, bool batch, bool post)
     const Tile *c = dir == DIR_X ? (batch ? xb : x1) : (batch ? yb : y1);
     int nc = dir == DIR_X ? (batch ? 6 : 5) : 4;
     for (int i = 0; i < nc; i++)
-        if (tile_ok(c[i], n, nl, dir, post)) return c[i];
+        if (tile_ok(c[i], n, nl, dir, post, src)) return c[i];
     return Tile{0, 0};
 }
 
@@ -186,8 +186,9 @@ int get_table(ccfv_ctx *c, Dir dir, double h, const SolveTable **out)
     // uniform-displacement path: locate face 0 of every line exactly as the
     // general path does; accept the sweep when every xi is >= 1e-9 from a
     // cell edge (then every face's floor() agrees with face 0's)
-    const bool uniform = per && (P.vel_kind == CCFV_VEL_SOLID_BODY ||
-                                 P.vel_kind == CCFV_VEL_CONSTANT);
+    // (a source term takes the general path)
+    const bool uniform = per && P.src_kind == CCFV_SRC_NONE &&
+                         (P.vel_kind == CCFV_VEL_SOLID_BODY || P.vel_kind == CCFV_VEL_CONSTANT);
     if (uniform && !getenv("CCFV_NO_FAST")) {
         const int nl = dir == DIR_X ? P.ny : P.nx;
         const double lo = dir == DIR_X ? P.x_lo : P.y_lo;
@@ -272,6 +273,10 @@ SweepArgs make_args(ccfv_ctx *c, Dir dir, int substep, const SolveTable &T,
     a.field0 = field0;
     a.substep = substep;
     a.flag = c->d_flag;
+    a.src_kind = P.src_kind;
+    for (int i = 0; i < 5; i++) a.src[i] = P.src[i];
+    // theta*h: 0.25*dt for x, 0.5*dt for y (T2S4_Parareal.cu:575, :607)
+    a.src_w = (dir == DIR_X ? 0.25 : 0.5) * h;
     return a;
 }
 
@@ -513,6 +518,10 @@ int validate_problem(const ccfv_problem *p)
         return fail(CCFV_EINVAL, "diffusion coefficient must be >= 0");
     if (p->vel_kind == CCFV_VEL_SWIRL && !(p->vel[0] > 0.0))
         return fail(CCFV_EINVAL, "swirl period must be > 0");
+    if (p->src_kind < CCFV_SRC_NONE || p->src_kind > CCFV_SRC_SINSIN)
+        return fail(CCFV_EINVAL, "unknown source kind");
+    for (int i = 0; i < 5; i++)
+        if (!std::isfinite(p->src[i])) return fail(CCFV_EINVAL, "source parameters must be finite");
     if (p->bc == CCFV_BC_PERIODIC && (p->flags & CCFV_FLAG_STEP3_POSTREMAP))
         return fail(CCFV_EINVAL,
                     "the third-sub-step post-remap derivative is a reference "
@@ -644,9 +653,10 @@ int ccfv_create(const ccfv_problem *p, ccfv_ctx **out)
     HIPCHK(hipGetDevice(&c->device));
     c->ncell = (size_t)p->nx * p->ny;
     const bool post = (p->flags & CCFV_FLAG_STEP3_POSTREMAP) != 0;
+    const bool src = p->src_kind != CCFV_SRC_NONE;
     for (int b = 0; b < 2; b++) {
-        c->tile[DIR_X][b] = pick_tile(DIR_X, p->nx, p->ny, b, post);
-        c->tile[DIR_Y][b] = pick_tile(DIR_Y, p->ny, p->nx, b, false);
+        c->tile[DIR_X][b] = pick_tile(DIR_X, p->nx, p->ny, b, post, src);
+        c->tile[DIR_Y][b] = pick_tile(DIR_Y, p->ny, p->nx, b, false, src);
     }
     c->tile[DIR_X][0] = env_tile("CCFV_TILE_X1", c->tile[DIR_X][0]);
     c->tile[DIR_X][1] = env_tile("CCFV_TILE_XB", c->tile[DIR_X][1]);
@@ -655,7 +665,7 @@ int ccfv_create(const ccfv_problem *p, ccfv_ctx **out)
     for (int d = 0; d < 2; d++)
         for (int b = 0; b < 2; b++)
             if (!tile_ok(c->tile[d][b], d == DIR_X ? p->nx : p->ny,
-                         d == DIR_X ? p->ny : p->nx, (Dir)d, post && d == DIR_X)) {
+                         d == DIR_X ? p->ny : p->nx, (Dir)d, post && d == DIR_X, src)) {
                 delete c;
                 return fail(CCFV_EINVAL, "no valid tile for this grid");
             }
@@ -909,7 +919,7 @@ int ccfv_set_tiles(ccfv_ctx *c, const int32_t t[8])
         v[i] = Tile{t[2 * i], t[2 * i + 1]};
         Dir d = i < 2 ? DIR_X : DIR_Y;
         int n = d == DIR_X ? c->P.nx : c->P.ny, nl = d == DIR_X ? c->P.ny : c->P.nx;
-        if (!tile_ok(v[i], n, nl, d, post && d == DIR_X))
+        if (!tile_ok(v[i], n, nl, d, post && d == DIR_X, c->P.src_kind != CCFV_SRC_NONE))
             return fail(CCFV_EINVAL, "invalid tile shape for this grid");
     }
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -969,7 +979,9 @@ int ccfv_autotune(ccfv_ctx *c, int32_t batch_fields, double h, int32_t out[8])
             for (int w : ws)
                 for (int nw : nws) {
                     Tile t{w, nw};
-                    if (!tile_ok(t, n, nl, dir, post && dir == DIR_X)) continue;
+                    if (!tile_ok(t, n, nl, dir, post && dir == DIR_X,
+                                 c->P.src_kind != CCFV_SRC_NONE))
+                        continue;
                     float tbest = 1e30f;
                     for (int trial = 0; trial < 2; trial++) {
                         hipError_t e = run_sweep(c, dir, false, EPI_NONE, a, nf, t);

@@ -12,9 +12,11 @@ namespace ccfv {
 
 // cs: the line in LDS (pidx layout), overwritten with the result.
 // aux: a line-sized LDS buffer owned by this wave.
+// fs: a second line-sized LDS buffer of this wave, for the source's earlier
+// cell averages (st: the source's times; used when a.src_kind != 0).
 template <int DIR, int BC, int CH, bool POST>
 __device__ void process_line(const SweepArgs &a, const Times tt, double *cs, double *aux,
-                             int line, int lane)
+                             double *fs, const SrcTimes st, int line, int lane)
 {
     const int n = a.n;
     const int NL = n / CH;
@@ -24,26 +26,49 @@ __device__ void process_line(const SweepArgs &a, const Times tt, double *cs, dou
     const LineGeo lg = line_geo(a, tt, line);
     double g[4] = {0.0, 0.0, 0.0, 0.0};
     if (BC == BC_REF) ghosts(cs, n, g);
+    const bool src = a.src_kind != CCFV_SRC_NONE;
+    double fg[4] = {0.0, 0.0, 0.0, 0.0};
+    if (src) {
+        // the source's earlier cell averages of this line (T2S4_Parareal.cu:
+        // 143-165), remapped below like the solution
+        if (active)
+            for (int k = 0; k < CH; k++) fs[pidx(base + k)] = src_avg<DIR>(a, base + k, line, st.t0);
+        __syncthreads();
+        if (BC == BC_REF) ghosts(fs, n, fg);
+    }
 
     double b[CH];
-    if (active) rhs_general<DIR, BC>(a, tt, cs, g, aux, base, CH, lg, inv12d, POST);
+    if (active)
+        rhs_general<DIR, BC>(a, tt, cs, g, aux, base, CH, lg, inv12d, POST,
+                             src ? fs : nullptr, fg, line, st.t1);
     if (POST) {
         // the reference's third sub-step takes the derivative of the remapped
-        // line (T2S4_Parareal.cu:929-932)
+        // line (T2S4_Parareal.cu:929-932); the source joins afterwards
         __syncthreads();
         double gs[4] = {0.0, 0.0, 0.0, 0.0};
         if (BC == BC_REF) ghosts(aux, n, gs);
         if (active) {
             double dl = deriv_ref<BC>(a, aux, gs, face_foot<DIR>(a, tt, base, lg),
                                       inv12d);
+            Face gl;
+            bool bad = false;
+            if (src) gl = eval_face<DIR, BC, true>(a, tt, fs, fg, base, lg, inv12d);
 #pragma unroll
             for (int k = 0; k < CH; k++) {
                 double dr = deriv_ref<BC>(a, aux, gs,
                                           face_foot<DIR>(a, tt, base + k + 1, lg),
                                           inv12d);
-                b[k] = aux[pidx(base + k)] + a.coef * (dr - dl);
+                double v = aux[pidx(base + k)];
+                if (src) {
+                    Face gr = eval_face<DIR, BC, true>(a, tt, fs, fg, base + k + 1, lg, inv12d);
+                    v = v + src_rhs<DIR, BC>(a, tt, fs, fg, lg, line, base + k, gl, gr, st.t1,
+                                             bad);
+                    gl = gr;
+                }
+                b[k] = v + a.coef * (dr - dl);
                 dl = dr;
             }
+            if (bad) *a.flag = 1;
         }
         __syncthreads();
     } else if (active) {
@@ -74,9 +99,11 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, int field, int 
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double *aux = lds + (W + wave) * LP;
+    double *fs = lds + (W + NW + wave) * LP;  // allocated with a source only
     const Times tt = sub_times(a, field);
+    const SrcTimes st = src_times(a, field, false);
     for (int l = wave; l < W; l += NW)
-        process_line<DIR_X, BC, CH, POST>(a, tt, lds + l * LP, aux, j0 + l, lane);
+        process_line<DIR_X, BC, CH, POST>(a, tt, lds + l * LP, aux, fs, st, j0 + l, lane);
     __syncthreads();
     if (!FUSE) {
         if (EPI == EPI_NONE) store_tile_x(a.out + foff, a.ld, n, j0, W, lds, LP);
@@ -88,8 +115,9 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, int field, int 
         __syncthreads();
     }
     const Times t2 = sub_times_next(a, field);
+    const SrcTimes st2 = src_times(a, field, true);
     for (int l = wave; l < W; l += NW)
-        process_line<DIR_X, BC, CH, false>(a, t2, lds + l * LP, aux, j0 + l, lane);
+        process_line<DIR_X, BC, CH, false>(a, t2, lds + l * LP, aux, fs, st2, j0 + l, lane);
     __syncthreads();
     store_tile_x(a.out2 + foff, a.ld, n, j0, W, lds, LP);
 }
@@ -105,9 +133,11 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs &a, int field, int 
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double *aux = lds + (W + wave) * LP;
+    double *fs = lds + (W + NW + wave) * LP;  // allocated with a source only
     const Times tt = sub_times(a, field);
+    const SrcTimes st = src_times(a, field, false);
     for (int l = wave; l < W; l += NW)
-        process_line<DIR_Y, BC, CH, false>(a, tt, lds + l * LP, aux, i0 + l, lane);
+        process_line<DIR_Y, BC, CH, false>(a, tt, lds + l * LP, aux, fs, st, i0 + l, lane);
     __syncthreads();
     store_tile_y(a.out + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
 }
@@ -159,10 +189,11 @@ __global__ __launch_bounds__(512) void ragged_y_kernel(const RaggedItem *items,
     sweep_y_body<BC, CH>(a, field, local - field * nt, W, NW, lds);
 }
 
-size_t sweep_lds_bytes_generic(int n, Tile tile)
+size_t sweep_lds_bytes_generic(int n, Tile tile, bool src)
 {
-    // W tile lines + one auxiliary line per wave
-    return sizeof(double) * (size_t)line_pitch(n) * (size_t)(tile.w + tile.nw);
+    // W tile lines + one auxiliary line per wave (+ one source line per wave)
+    return sizeof(double) * (size_t)line_pitch(n) *
+           (size_t)(tile.w + tile.nw * (src ? 2 : 1));
 }
 
 template <typename K>
@@ -199,7 +230,7 @@ static hipError_t dispatch(Dir dir, bool post, Epi epi, bool fuse, const SweepAr
                            int nf, Tile t, hipStream_t s)
 {
     dim3 grid(a.nl / t.w, nf), block(64 * t.nw);
-    size_t lds = sweep_lds_bytes_generic(a.n, t);
+    size_t lds = sweep_lds_bytes_generic(a.n, t, a.src_kind != CCFV_SRC_NONE);
     if (dir == DIR_Y) return launch(sweep_y_kernel<BC, CH>, grid, block, lds, s, a, t);
     if (BC == BC_REF && post)
         return fuse ? dispatch_x<BC, CH, true, true>(epi, grid, block, lds, s, a, t)

@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_layouts():
-    assert ctypes.sizeof(T.Problem) == 96
+    assert ctypes.sizeof(T.Problem) == 96 + 8 + 5 * 8
     assert ctypes.sizeof(T.IC) == 8 + 16 * 8
     assert ctypes.sizeof(T.ParaReport) == 16 + 64 * 8 + 24
     assert ctypes.sizeof(T.KernelStat) == 24
@@ -49,7 +49,7 @@ def test_reference_presets_match_the_c_side():
     q = T.reference_problem(128, 0.01)
     for f, _ in T.Problem._fields_:
         a, b = getattr(p, f), getattr(q, f)
-        if f == "vel":
+        if f in ("vel", "src"):
             assert list(a) == list(b)
         else:
             assert a == b, f
@@ -66,6 +66,8 @@ def test_reference_presets_match_the_c_side():
     ({"vel_kind": 9}, "velocity"),
     ({"x_hi": -2.0}, "box"),
     ({"flags": T.FLAG_STEP3_POSTREMAP}, "reference"),
+    ({"src_kind": 3}, "source kind"),
+    ({"src_kind": T.SRC_CONSTANT, "src": (float("nan"),)}, "source parameters"),
 ])
 def test_create_rejects_bad_problems(kw, msg):
     p = T.rotation_problem(64).copy(**kw)

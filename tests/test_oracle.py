@@ -337,3 +337,102 @@ def test_parareal_slices_up_to_k_are_exact(oracle):
     for s in range(4):
         assert np.linalg.norm(r["slices"][s] - ser[s]) <= 1e-13 * np.linalg.norm(ser[s])
     assert np.linalg.norm(r["slices"][ns] - ser[ns]) > 1e-10 * np.linalg.norm(ser[ns])
+
+
+# ---------------------------------------------------------------------------
+# source term: f (T2S4_Parareal.cu:58-61), its Simpson cell average
+# (:142-163) and the sub-step times (:839-844 x, :887-891 y, :934-940 x)
+# ---------------------------------------------------------------------------
+def _still(n, bc=T.BC_PERIODIC, flags=0):
+    """V = 0, nu = 0: the feet sit on the faces, the remap returns its input
+    and the compact system is inverted exactly, so a step only adds the
+    source."""
+    return T.make_problem(n, n, bc=bc, vel_kind=T.VEL_CONSTANT, vel=(0.0, 0.0),
+                          nu=0.0, flags=flags)
+
+
+@pytest.mark.parametrize("bc,flags", [(T.BC_PERIODIC, 0), (T.BC_REFERENCE, 0),
+                                      (T.BC_REFERENCE, T.FLAGS_REFERENCE)])
+def test_constant_source_adds_two_h_f_per_step(oracle, bc, flags):
+    # every sub-step adds theta h (fbar0 + f1): 2 (1/4)(h) + (1/2)(2 h) = 2 h
+    p = _still(16, bc, flags)
+    p.src_kind = T.SRC_CONSTANT
+    p.src[0] = 0.7
+    u0 = np.full(256, 1.25)
+    u1 = oracle.step(p, u0, 0.3, 0.01)
+    # rounding of the line solves on values near 1.26; a wrong weight is O(1e-3)
+    assert np.abs(u1 - (u0 + 2 * 0.01 * 0.7)).max() <= 1e-13
+    if bc != T.BC_PERIODIC:
+        return  # the closed box's edge rows act as a zero wall (:556-591)
+    # with transport and diffusion a periodic uniform field stays uniform
+    q = p.copy(vel=(0.8, -0.3), nu=0.02)
+    u1 = oracle.step(q, u0, 0.3, 0.01)
+    assert np.abs(u1 - (u0 + 2 * 0.01 * 0.7)).max() <= 1e-13
+
+
+def test_source_times_and_simpson_average_closed_form(oracle):
+    """One step on a still field adds h [3/4 f(tn) + 1/4 f(tn + h/2)
+    + f(tn + h)] (x: f(tn) and f(tn+h/2); y: f(tn) and f(tn+h); last x:
+    f(tn+h) twice).  Simpson's average of sin(k x) or cos(k x) over a cell of
+    half-width d is its centre value times (2 + cos(k d)) / 3."""
+    n, tn, h = 24, 0.37, 0.01
+    w, s = 3.1, (0.9, -0.4, 0.25, 0.6)
+    p = _still(n)
+    p.src_kind = T.SRC_SINSIN
+    p.src[0] = w
+    for i, v in enumerate(s):
+        p.src[i + 1] = v
+    k, d = 2 * np.pi, 0.5 / n
+    xc = (np.arange(n) + 0.5) / n
+    q = (2 + np.cos(k * d)) / 3
+    S, C = q * np.sin(k * xc), q * np.cos(k * xc)
+    SS, CS, SC = np.outer(S, S), np.outer(C, S), np.outer(S, C)  # [i (x), j (y)]
+
+    def favg(t):
+        return ((s[0] * np.cos(w * t) + s[1] * np.sin(w * t)) * SS
+                + np.sin(w * t) * (s[2] * CS + s[3] * SC))
+
+    u0 = np.zeros(n * n)
+    gain = oracle.step(p, u0, tn, h).reshape(n, n)
+    want = h * (0.75 * favg(tn) + 0.25 * favg(tn + 0.5 * h) + favg(tn + h))
+    assert np.abs(gain - want).max() <= 1e-16  # the gain is O(1e-2)
+    # the symmetric (second-order) weights are not what the scheme does
+    sym = h * (favg(tn) + favg(tn + h))
+    assert np.abs(gain - sym).max() > 1e-6
+
+
+def _manufactured_error(oracle, n, nt, t1=0.25, scale=1.0):
+    p, ic, amp = T.manufactured_sinsin(n)
+    for i in range(1, 5):
+        p.src[i] *= scale
+    u = oracle.plain(p, oracle.exact(p, ic, 0.0), 0.0, t1, nt)
+    d = 1.0 / n
+    x = (np.arange(n) + 0.5) * d
+    a = (np.cos(2 * np.pi * (x - 0.5 * d)) - np.cos(2 * np.pi * (x + 0.5 * d))) / (2 * np.pi * d)
+    exact = oracle.exact(p, ic, t1) + amp(t1) * np.outer(a, a).reshape(-1)
+    return math.sqrt(float(np.mean((u - exact) ** 2)))
+
+
+def test_manufactured_source_convergence(oracle):
+    """u_t + V.grad u = nu lap u + g with the exact solution translated
+    sin*sin + A sin(w t) S(x) S(y), run with f = g / 2.  The sub-step times
+    give a source quadrature whose first moment is off by h^2 / 16 per step,
+    so the error falls at first order in h and at fourth order in space once
+    h is small; with f = g the error does not fall at all."""
+    e = [_manufactured_error(oracle, 64, nt) for nt in (64, 128, 256)]
+    for a, b in zip(e, e[1:]):
+        assert 1.9 <= a / b <= 2.1, e
+    e = [_manufactured_error(oracle, n, 4096) for n in (16, 32)]
+    assert e[0] / e[1] >= 14, e
+    assert _manufactured_error(oracle, 32, 128, scale=2.0) > 0.2
+
+
+def test_no_source_is_unchanged_by_zero_parameters(oracle):
+    p, ic = T.rotation_problem(32), T.rotation_ic()
+    u0 = oracle.exact(p, ic, 0.0)
+    a = oracle.step(p, u0, 0.1, 1.0 / 64)
+    q = p.copy()
+    q.src_kind = T.SRC_CONSTANT
+    q.src[0] = 0.0
+    b = oracle.step(q, u0, 0.1, 1.0 / 64)
+    assert np.array_equal(a, b)
