@@ -51,7 +51,7 @@ struct SweepArgs {
     double eps;         // remap clip epsilon (closed box)
     double coef, ca, diag0, vl, smden;
     const double *tab;
-    const double *lineS, *lineK;  // uniform path: shift and weights per line
+    const double *lineS, *lineK;  // uniform path: shift and 6 filter weights per line
     int vel_kind;
     double v0, v1, v2;  // velocity parameters
     // time: field f of the launch sits at tn = t0 + (field0 + f)*tDT, the

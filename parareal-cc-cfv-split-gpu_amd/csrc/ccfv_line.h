@@ -21,6 +21,10 @@ constexpr int BC_PER = CCFV_BC_PERIODIC;
 // patterns free of bank conflicts.
 __device__ __forceinline__ int pidx(int e) { return e + (e >> 3); }
 __host__ __device__ inline int line_pitch(int n) { return n + (n >> 3) + 1; }
+// ghost cells after the last cell of a line in the uniform-offset sweep of
+// lanes with ch >= 8 cells: a window of ch + 5 cells starting up to 7 cells
+// into a chunk reaches (ch + 11) / 8 chunks past the lane's first
+__host__ __device__ inline int fast_ghost_cells(int ch) { return ch >= 8 ? 8 * ((ch + 11) >> 3) : 0; }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi)
 {
@@ -465,10 +469,11 @@ __device__ __forceinline__ int ragged_item(const int *prefix, int nitems, int b)
 // compact operator and implicit half, shared by both paths
 // ---------------------------------------------------------------------------
 // b: RHS of this lane's CH cells.  Tables: mneg = -a*den, den = Thomas
-// pivots' inverses, pneg = -p, z = T^{-1} u (Sherman-Morrison), mf/mb =
-// per-lane chunk products of mneg/pneg (identity 1 on idle lanes).  LDSTAB:
-// mneg..z are LDS lines (pidx layout) rather than global arrays.  The result
-// overwrites the lane's cells of cs.
+// pivots' inverses over 12 (the compact operator's 1/12 taken with them),
+// pneg = -p, z = T^{-1} u (Sherman-Morrison), mf/mb = per-lane chunk
+// products of mneg/pneg (identity 1 on idle lanes).  LDSTAB: mneg..z are LDS
+// lines (pidx layout) rather than global arrays.  The result overwrites the
+// lane's cells of cs.
 template <int BC, int CH, bool LDSTAB>
 __device__ __forceinline__ void solve_tail(const SweepArgs &a, double (&b)[CH],
                                            int lane, bool active,
@@ -492,7 +497,7 @@ __device__ __forceinline__ void solve_tail(const SweepArgs &a, double (&b)[CH],
     for (int k = 0; k < CH; k++) {
         double bn = k + 1 < CH ? b[k + 1] : bnext;
         double bp = k > 0 ? b[k - 1] : bprev;
-        r[k] = (bn + 10.0 * b[k] + bp) * (1.0 / 12.0);
+        r[k] = bn + 10.0 * b[k] + bp;  // the 1/12 is in den
     }
 
     // forward pass q_e = r_e den_e + mneg_e q_{e-1}: chunk pass, lane scan
@@ -591,6 +596,33 @@ __device__ __forceinline__ void load_tile_x(const double *in, int ld, int n,
                                             int j0, int W, double *lds, int LP)
 {
     const int hw = W >> 1, sh = log2i(hw);
+    const int R = (int)blockDim.x >> sh;  // rows per pass of the block
+    if ((R & 7) == 0 && n % R == 0) {
+        // Each work-item keeps its column pair and moves rows i0, i0+R, ...:
+        // global address = uniform row offset + fixed item offset, LDS
+        // address = item base + (R + R/8) per pass (pidx of R rows later).
+        const int c2 = threadIdx.x & (hw - 1), i0 = threadIdx.x >> sh;
+        const unsigned lo = (unsigned)(i0 * ld + 2 * c2);
+        double *d0 = lds + 2 * c2 * LP + pidx(i0);
+        double *d1 = d0 + LP;
+        const int nr = n / R, ls = R + (R >> 3);
+        for (int p = 0; p < nr; p += TILE_UNROLL) {
+            double2 v[TILE_UNROLL];
+#pragma unroll
+            for (int u = 0; u < TILE_UNROLL; u++)
+                if (p + u < nr) {
+                    const double *row = in + j0 + (size_t)(p + u) * R * ld;
+                    v[u] = *reinterpret_cast<const double2 *>(row + lo);
+                }
+#pragma unroll
+            for (int u = 0; u < TILE_UNROLL; u++)
+                if (p + u < nr) {
+                    d0[(p + u) * ls] = v[u].x;
+                    d1[(p + u) * ls] = v[u].y;
+                }
+        }
+        return;
+    }
     const int total = hw * n, step = blockDim.x;
     for (int b0 = threadIdx.x; b0 < total; b0 += TILE_UNROLL * step) {
         double2 v[TILE_UNROLL];
@@ -618,6 +650,23 @@ __device__ __forceinline__ void store_tile_x(double *out, int ld, int n, int j0,
                                              int W, const double *lds, int LP)
 {
     const int hw = W >> 1, sh = log2i(hw);
+    const int R = (int)blockDim.x >> sh;
+    if ((R & 7) == 0 && n % R == 0) {
+        // the addressing of load_tile_x
+        const int c2 = threadIdx.x & (hw - 1), i0 = threadIdx.x >> sh;
+        const unsigned lo = (unsigned)(i0 * ld + 2 * c2);
+        const double *d0 = lds + 2 * c2 * LP + pidx(i0);
+        const double *d1 = d0 + LP;
+        const int nr = n / R, ls = R + (R >> 3);
+        for (int p = 0; p < nr; p++) {
+            double2 v;
+            v.x = d0[p * ls];
+            v.y = d1[p * ls];
+            double *row = out + j0 + (size_t)p * R * ld;
+            *reinterpret_cast<double2 *>(row + lo) = v;
+        }
+        return;
+    }
     for (int idx = threadIdx.x; idx < hw * n; idx += blockDim.x) {
         int c2 = idx & (hw - 1), i = idx >> sh;
         double2 v;
@@ -727,6 +776,43 @@ __device__ __forceinline__ void load_tile_y(const double *in, int ld, int n,
 {
     const int hn = n >> 1;
     const int total = W * hn, step = blockDim.x;
+    if (hn % step == 0) {
+        // Work-item t moves double2 t, t+step, ... of every line: global
+        // address = uniform (line, pass) offset + 2t, LDS address = pidx(2t)
+        // + uniform offset (2 step + step/4 per pass: step is a multiple of 64).
+        const int QP = hn / step, tot = W * QP, t = threadIdx.x;
+        double *dl = lds + pidx(2 * t);
+        const int qs = 2 * step + (step >> 2);
+        for (int e0 = 0; e0 < tot; e0 += TILE_UNROLL) {
+            const int l0 = e0 / QP, q0 = e0 - l0 * QP;
+            double2 v[TILE_UNROLL];
+            int l = l0, q = q0;
+#pragma unroll
+            for (int u = 0; u < TILE_UNROLL; u++)
+                if (e0 + u < tot) {
+                    const double *row = in + (size_t)l * ld + 2 * q * step;
+                    v[u] = *reinterpret_cast<const double2 *>(row + 2 * t);
+                    if (++q == QP) {
+                        q = 0;
+                        ++l;
+                    }
+                }
+            l = l0;
+            q = q0;
+#pragma unroll
+            for (int u = 0; u < TILE_UNROLL; u++)
+                if (e0 + u < tot) {
+                    double *d = dl + l * LP + q * qs;
+                    d[0] = v[u].x;
+                    d[1] = v[u].y;
+                    if (++q == QP) {
+                        q = 0;
+                        ++l;
+                    }
+                }
+        }
+        return;
+    }
     for (int b0 = threadIdx.x; b0 < total; b0 += TILE_UNROLL * step) {
         double2 v[TILE_UNROLL];
 #pragma unroll
@@ -755,6 +841,23 @@ __device__ __forceinline__ void store_tile_y(double *out, int ld, int n, int W,
                                              const double *lds, int LP)
 {
     const int hn = n >> 1;
+    const int step = blockDim.x;
+    if (hn % step == 0) {
+        // the addressing of load_tile_y
+        const int QP = hn / step, t = threadIdx.x;
+        const double *dl = lds + pidx(2 * t);
+        const int qs = 2 * step + (step >> 2);
+        for (int l = 0; l < W; l++)
+            for (int q = 0; q < QP; q++) {
+                const double *d = dl + l * LP + q * qs;
+                double2 v;
+                v.x = d[0];
+                v.y = d[1];
+                double *row = out + (size_t)l * ld + 2 * q * step;
+                *reinterpret_cast<double2 *>(row + 2 * t) = v;
+            }
+        return;
+    }
     for (int idx = threadIdx.x; idx < W * hn; idx += blockDim.x) {
         int r, j2;
         split_row(idx, hn, r, j2);

@@ -166,7 +166,9 @@ int get_table(ccfv_ctx *c, Dir dir, double h, const SolveTable **out)
     }
     std::vector<double> tab(4 * (size_t)n + 128, 1.0);
     std::copy(mneg.begin(), mneg.end(), tab.begin());
-    std::copy(den.begin(), den.end(), tab.begin() + n);
+    // the device solves take the compact operator's 1/12 with the pivots:
+    // r = b_{i-1} + 10 b_i + b_{i+1}, first forward value r * (den / 12)
+    for (int i = 0; i < n; i++) tab[n + i] = den[i] / 12.0;
     std::copy(pneg.begin(), pneg.end(), tab.begin() + 2 * n);
     std::copy(z.begin(), z.end(), tab.begin() + 3 * n);
     {
@@ -197,7 +199,7 @@ int get_table(ccfv_ctx *c, Dir dir, double h, const SolveTable **out)
         const double tau = dir == DIR_X ? 0.5 * h : h;
         const double inv12d = 1.0 / (12.0 * d);
         VelParams v{P.vel_kind, P.vel[0], P.vel[1], P.vel[2]};
-        std::vector<double> ld(6 * (size_t)nl);
+        std::vector<double> ld(7 * (size_t)nl);
         bool ok = true;
         for (int L = 0; L < nl && ok; L++) {
             double o0 = corner(olo, od, L), o1 = corner(olo, od, L + 1);
@@ -211,7 +213,17 @@ int get_table(ccfv_ctx *c, Dir dir, double h, const SolveTable **out)
                 break;
             }
             ld[L] = (double)m;
-            fir_weights(xi, T.coef, inv12d, &ld[nl + 5 * (size_t)L]);
+            // b_i = C_m + K_{m+1} - K_m with K_m = sum_s kap_s C_{m-2+s}
+            // (m = i + shift) as one 6-tap filter over C_{m-2} .. C_{m+3}
+            double kap[5];
+            fir_weights(xi, T.coef, inv12d, kap);
+            double *w = &ld[nl + 6 * (size_t)L];
+            w[0] = -kap[0];
+            w[1] = kap[0] - kap[1];
+            w[2] = kap[1] - kap[2] + 1.0;
+            w[3] = kap[2] - kap[3];
+            w[4] = kap[3] - kap[4];
+            w[5] = kap[4];
         }
         if (ok) {
             HIPCHK(hipMalloc(&T.line_dev, sizeof(double) * ld.size()));

@@ -6,11 +6,11 @@
 // in cell q+S.  The host locates face 0 of every line with the same
 // contraction-free arithmetic as the general path and accepts the line only
 // when xi is at least 1e-9 away from a cell edge, so each face's floor()
-// agrees; it stores S and the five filter weights kappa(xi) per line.  The
-// PPM antiderivative, remap and foot derivative (T2S4_Parareal.cu:260-547)
-// then collapse to
+// agrees.  The PPM antiderivative, remap and foot derivative
+// (T2S4_Parareal.cu:260-547) then collapse to
 //     b_i = C_{i+S} + K_{i+S+1} - K_{i+S},   K_m = sum_s kappa_s C_{m-2+s},
-// followed by the compact operator and the tridiagonal solve.
+// i.e. one 6-tap filter b_i = sum_j w_j C_{i+S-2+j}; the host stores S and
+// w(xi) per line.  The compact operator and the tridiagonal solve follow.
 #include "ccfv_line.h"
 
 namespace ccfv {
@@ -58,20 +58,32 @@ struct LaneTab<CH, false> {
 // a line's cell shift and filter weights
 struct LineK {
     int S;
-    double k0, k1, k2, k3, k4;
+    double w0, w1, w2, w3, w4, w5;
 };
 
 __device__ __forceinline__ LineK line_k(const SweepArgs &a, int line)
 {
     LineK l;
     l.S = (int)a.lineS[line];
-    const double *kp = a.lineK + 5 * (size_t)line;
-    l.k0 = kp[0];
-    l.k1 = kp[1];
-    l.k2 = kp[2];
-    l.k3 = kp[3];
-    l.k4 = kp[4];
+    const double *wp = a.lineK + 6 * (size_t)line;
+    l.w0 = wp[0];
+    l.w1 = wp[1];
+    l.w2 = wp[2];
+    l.w3 = wp[3];
+    l.w4 = wp[4];
+    l.w5 = wp[5];
     return l;
+}
+
+// b_q = sum_j w_j C_{q+j} over a window starting two cells before the shift
+template <int CH>
+__device__ __forceinline__ void filter6(const LineK &lk, const double (&cw)[CH + 5],
+                                        double (&b)[CH])
+{
+#pragma unroll
+    for (int q = 0; q < CH; q++)
+        b[q] = lk.w0 * cw[q] + lk.w1 * cw[q + 1] + lk.w2 * cw[q + 2] + lk.w3 * cw[q + 3] +
+               lk.w4 * cw[q + 4] + lk.w5 * cw[q + 5];
 }
 
 template <int CH, bool REG>
@@ -84,7 +96,6 @@ __device__ __forceinline__ void fast_line(const SweepArgs &a, double *cs,
     const bool active = lane < NL;
     const int base = lane * CH;
     const int S = lk.S;
-    const double k0 = lk.k0, k1 = lk.k1, k2 = lk.k2, k3 = lk.k3, k4 = lk.k4;
     double b[CH];
     if (active) {
         double cw[CH + 5];
@@ -96,14 +107,7 @@ __device__ __forceinline__ void fast_line(const SweepArgs &a, double *cs,
             cw[t] = cs[pidx(k)];
             k = k + 1 == n ? 0 : k + 1;
         }
-        double Kp = k0 * cw[0] + k1 * cw[1] + k2 * cw[2] + k3 * cw[3] + k4 * cw[4];
-#pragma unroll
-        for (int q = 0; q < CH; q++) {
-            double Kn = k0 * cw[q + 1] + k1 * cw[q + 2] + k2 * cw[q + 3] +
-                        k3 * cw[q + 4] + k4 * cw[q + 5];
-            b[q] = cw[q + 2] + (Kn - Kp);
-            Kp = Kn;
-        }
+        filter6<CH>(lk, cw, b);
     } else {
 #pragma unroll
         for (int q = 0; q < CH; q++) b[q] = 0.0;
@@ -121,7 +125,7 @@ __device__ __forceinline__ void fast_line(const SweepArgs &a, double *cs,
     for (int k = 0; k < CH; k++) {
         double bn = k + 1 < CH ? b[k + 1] : bnext;
         double bp = k > 0 ? b[k - 1] : bprev;
-        r[k] = (bn + 10.0 * b[k] + bp) * (1.0 / 12.0);
+        r[k] = bn + 10.0 * b[k] + bp;  // the 1/12 is in den
     }
 
     // forward pass q_e = r_e den_e + mneg_e q_{e-1} (mneg_0 = 0)
@@ -191,6 +195,186 @@ __device__ __forceinline__ void fast_line(const SweepArgs &a, double *cs,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Lines of 8 or more cells per lane (n > 256).  A lane's cells are one or
+// more whole pidx chunks of 8, so the window of every lane of a line starts
+// at the same offset r within a chunk: element t of lane L's window sits at
+// 9 Cb(L) + pidx(r + t), a per-lane base plus a wave-uniform offset.  Windows that run
+// past the last cell read ghost chunks after it, refreshed from the line's
+// first cells before each sweep, so no address wraps.  When the line fills
+// the wave (n = 64 CH) the lane-neighbour moves are DPP wave shifts.
+// ---------------------------------------------------------------------------
+// lane i takes lane i-1 (shr) or i+1 (shl), 0 where that lane is off the
+// wave; rotates wrap around the wave
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x)
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+constexpr int DPP_WAVE_SHL1 = 0x130, DPP_WAVE_ROL1 = 0x134;
+constexpr int DPP_WAVE_SHR1 = 0x138, DPP_WAVE_ROR1 = 0x13C;
+
+__device__ __forceinline__ double readlane_d(double x, int l)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+    return __hiloint2double(hi, lo);
+}
+
+// the lane's window: cells Cb(L) + pidx(r + t), r wave-uniform (a scalar
+// offset added to one per-lane base)
+template <int CH>
+__device__ __forceinline__ void load_window(const double *p, int r, double (&cw)[CH + 5])
+{
+#pragma unroll
+    for (int t = 0; t < CH + 5; t++) cw[t] = p[(r + t) + ((r + t) >> 3)];
+}
+
+// pf / pl: the back-substitution factor of the lane's first / last cell
+// (pneg_0 on lane 0, 0 on the last lane, mneg elsewhere)
+template <int CH, bool REG>
+__device__ __forceinline__ void fast_line_chunked(const SweepArgs &a, double *cs,
+                                                  const LineK &lk, int lane,
+                                                  const LaneTab<CH, REG> &tb, double mfl,
+                                                  double mbl, double pf, double pl)
+{
+    static_assert(CH >= 8, "chunk-aligned lanes");
+    const int n = a.n, NL = n / CH, NC = n >> 3;
+    const bool full = NL == 64;
+    const bool active = lane < NL;
+    const int base = lane * CH;
+
+    // ghost chunks: cells [0, FAST_GHOST) again after cell n-1
+    if (lane < fast_ghost_cells(CH)) cs[pidx(lane) + 9 * NC] = cs[pidx(lane)];
+
+    int s2 = (lk.S - 2) % n;
+    s2 = s2 < 0 ? s2 + n : s2;
+    int cb = lane * (CH / 8) + (s2 >> 3);
+    cb = cb >= NC ? cb - NC : cb;
+    double b[CH];
+    if (active) {
+        double cw[CH + 5];
+        load_window<CH>(cs + 9 * cb, s2 & 7, cw);
+        filter6<CH>(lk, cw, b);
+    } else {
+#pragma unroll
+        for (int q = 0; q < CH; q++) b[q] = 0.0;
+    }
+
+    // compact operator, periodic (T2S4_Parareal.cu:616-654 with wrap-around)
+    double bprev, bnext;
+    if (full) {
+        bprev = dpp_d<DPP_WAVE_ROR1>(b[CH - 1]);
+        bnext = dpp_d<DPP_WAVE_ROL1>(b[0]);
+    } else {
+        bprev = __shfl_up(b[CH - 1], 1);
+        bnext = __shfl_down(b[0], 1);
+        const double wrap_last = readlane_d(b[CH - 1], NL - 1);
+        const double wrap_first = readlane_d(b[0], 0);
+        if (lane == 0) bprev = wrap_last;
+        if (lane == NL - 1) bnext = wrap_first;
+    }
+    double r[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        double bn = k + 1 < CH ? b[k + 1] : bnext;
+        double bp = k > 0 ? b[k - 1] : bprev;
+        r[k] = bn + 10.0 * b[k] + bp;  // the 1/12 is in den
+    }
+
+    // forward pass q_e = r_e den_e + mneg_e q_{e-1} (mneg_0 = 0)
+    double M = mfl, Q = 0.0;
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            double c = r[k] * tb.D(k, base + k);
+            r[k] = c;
+            Q = c + tb.M(k, base + k) * Q;
+        }
+    }
+    // lane scan of the chunk maps (Q, M).  A DPP source off the wave reads
+    // (0, 0): Q is unchanged and M becomes 0, and such an M only ever scales
+    // carries from before lane 0, which are 0, so the scan is exact.
+    for (int st = 0, off = 1; st < a.sf; st++, off <<= 1) {
+        if (full && off == 1) {
+            const double Mp = dpp_d<DPP_WAVE_SHR1>(M);
+            const double Qp = dpp_d<DPP_WAVE_SHR1>(Q);
+            Q = M * Qp + Q;
+            M = M * Mp;
+        } else {
+            double Mp = __shfl_up(M, off), Qp = __shfl_up(Q, off);
+            if (lane >= off) {
+                Q = M * Qp + Q;
+                M = M * Mp;
+            }
+        }
+    }
+    double carry;
+    if (full) {
+        carry = dpp_d<DPP_WAVE_SHR1>(Q);
+    } else {
+        carry = __shfl_up(Q, 1);
+        if (lane == 0) carry = 0.0;
+    }
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            carry = r[k] + tb.M(k, base + k) * carry;
+            r[k] = carry;
+        }
+    }
+
+    // back substitution x_e = q_e + pneg_e x_{e+1}
+    M = mbl;
+    Q = 0.0;
+    if (active) {
+#pragma unroll
+        for (int k = CH - 1; k >= 0; k--) {
+            const double pn = k == 0 ? pf : (k == CH - 1 ? pl : tb.M(k, base + k));
+            Q = r[k] + pn * Q;
+        }
+    }
+    for (int st = 0, off = 1; st < a.sb; st++, off <<= 1) {
+        if (full && off == 1) {
+            const double Mn = dpp_d<DPP_WAVE_SHL1>(M);
+            const double Qn = dpp_d<DPP_WAVE_SHL1>(Q);
+            Q = M * Qn + Q;
+            M = M * Mn;
+        } else {
+            double Mn = __shfl_down(M, off), Qn = __shfl_down(Q, off);
+            if (lane + off < 64) {
+                Q = M * Qn + Q;
+                M = M * Mn;
+            }
+        }
+    }
+    if (full) {
+        carry = dpp_d<DPP_WAVE_SHL1>(Q);
+    } else {
+        carry = __shfl_down(Q, 1);
+        if (lane == 63) carry = 0.0;
+    }
+    if (active) {
+#pragma unroll
+        for (int k = CH - 1; k >= 0; k--) {
+            const double pn = k == 0 ? pf : (k == CH - 1 ? pl : tb.M(k, base + k));
+            carry = r[k] + pn * carry;
+            r[k] = carry;
+        }
+    }
+
+    // Sherman-Morrison: x = y - ((y_0 + vl y_{n-1}) / (1 + v.z)) z
+    const double y0 = readlane_d(r[0], 0);
+    const double yn = readlane_d(r[CH - 1], NL - 1);
+    const double fac = (y0 + a.vl * yn) / a.smden;
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) cs[pidx(base + k)] = r[k] - fac * tb.Z(k, base + k);
+    }
+}
+
 // The wave's lines l0 = wave, wave+NW, ... of a tile whose first line
 // starts at line index L0; each line's parameters are loaded while the
 // previous line is swept.
@@ -201,11 +385,27 @@ __device__ __forceinline__ void sweep_lines(const SweepArgs &a, double *lds, int
                                             double mfl, double mbl, double pneg0)
 {
     LineK cur = first;
-    for (int l = wave; l < W; l += NW) {
-        LineK nxt = cur;
-        if (l + NW < W) nxt = line_k(a, L0 + l + NW);
-        fast_line<CH, REG>(a, lds + l * LP, cur, lane, tb, mfl, mbl, pneg0);
-        cur = nxt;
+    if constexpr (CH >= 8) {
+        // back-substitution factors of the lane's end cells
+        const int NL = a.n / CH;
+        double pf = 0.0, pl = 0.0;
+        if (lane < NL) {
+            pf = lane == 0 ? pneg0 : tb.M(0, lane * CH);
+            pl = lane == NL - 1 ? 0.0 : tb.M(CH - 1, lane * CH + CH - 1);
+        }
+        for (int l = wave; l < W; l += NW) {
+            LineK nxt = cur;
+            if (l + NW < W) nxt = line_k(a, L0 + l + NW);
+            fast_line_chunked<CH, REG>(a, lds + l * LP, cur, lane, tb, mfl, mbl, pf, pl);
+            cur = nxt;
+        }
+    } else {
+        for (int l = wave; l < W; l += NW) {
+            LineK nxt = cur;
+            if (l + NW < W) nxt = line_k(a, L0 + l + NW);
+            fast_line<CH, REG>(a, lds + l * LP, cur, lane, tb, mfl, mbl, pneg0);
+            cur = nxt;
+        }
     }
 }
 
@@ -217,7 +417,7 @@ __device__ __forceinline__ void fast_x_body(const SweepArgs &a, int field, int t
                                            int W, int NW, double *lds)
 {
     constexpr bool REG = CH <= 8;
-    const int n = a.n, LP = line_pitch(n);
+    const int n = a.n, LP = line_pitch(n + fast_ghost_cells(CH));
     const size_t foff = (size_t)field * (size_t)a.fstride;
     const int j0 = tile * W;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -254,7 +454,7 @@ __device__ __forceinline__ void fast_y_body(const SweepArgs &a, int field, int t
                                            int W, int NW, double *lds)
 {
     constexpr bool REG = CH <= 8;
-    const int n = a.n, LP = line_pitch(n);
+    const int n = a.n, LP = line_pitch(n + fast_ghost_cells(CH));
     const size_t foff = (size_t)field * (size_t)a.fstride;
     const int i0 = tile * W;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -319,7 +519,8 @@ __global__ __launch_bounds__(256) void ragged_fast_y(const RaggedItem *items,
 
 size_t sweep_lds_bytes_fast(int n, Tile tile)
 {
-    return sizeof(double) * (size_t)line_pitch(n) * (size_t)tile.w;
+    return sizeof(double) * (size_t)line_pitch(n + fast_ghost_cells(chunk_for(n))) *
+           (size_t)tile.w;
 }
 
 template <typename K>
