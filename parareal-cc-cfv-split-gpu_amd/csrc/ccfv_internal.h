@@ -49,7 +49,8 @@ struct SweepArgs {
     double lo, hi, d;   // along-line box and cell width
     double olo, od;     // other-direction box start and cell width
     double eps;         // remap clip epsilon (closed box)
-    double coef, ca, diag0, vl, smden;
+    double coef, ca, diag0, vl;
+    double smrcp;       // Sherman-Morrison: 1 / (1 + v.z)
     const double *tab;
     const double *lineS, *lineK;  // uniform path: shift and 6 filter weights per line
     int vel_kind;

@@ -557,7 +557,7 @@ __device__ __forceinline__ void solve_tail(const SweepArgs &a, double (&b)[CH],
         // Sherman-Morrison: x = y - ((y_0 + vl y_{n-1}) / (1 + v.z)) z
         double y0 = __shfl(r[0], 0);
         double yn = __shfl(r[CH - 1], NL - 1);
-        double fac = (y0 + a.vl * yn) / a.smden;
+        double fac = (y0 + a.vl * yn) * a.smrcp;
         if (active) {
 #pragma unroll
             for (int k = 0; k < CH; k++) r[k] = r[k] - fac * CCFV_TAB(zt, base + k);

@@ -270,7 +270,7 @@ SweepArgs make_args(ccfv_ctx *c, Dir dir, int substep, const SolveTable &T,
     a.ca = T.ca;
     a.diag0 = T.diag0;
     a.vl = T.vl;
-    a.smden = T.smden;
+    a.smrcp = 1.0 / T.smden;
     a.tab = T.dev;
     a.lineS = T.line_dev;
     a.lineK = T.line_dev ? T.line_dev + a.nl : nullptr;

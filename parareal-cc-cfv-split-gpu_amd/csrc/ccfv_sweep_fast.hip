@@ -188,7 +188,7 @@ __device__ __forceinline__ void fast_line(const SweepArgs &a, double *cs,
     // Sherman-Morrison: x = y - ((y_0 + vl y_{n-1}) / (1 + v.z)) z
     double y0 = __shfl(r[0], 0);
     double yn = __shfl(r[CH - 1], NL - 1);
-    double fac = (y0 + a.vl * yn) / a.smden;
+    double fac = (y0 + a.vl * yn) * a.smrcp;
     if (active) {
 #pragma unroll
         for (int k = 0; k < CH; k++) cs[pidx(base + k)] = r[k] - fac * tb.Z(k, base + k);
@@ -233,17 +233,18 @@ __device__ __forceinline__ void load_window(const double *p, int r, double (&cw)
 }
 
 // pf / pl: the back-substitution factor of the lane's first / last cell
-// (pneg_0 on lane 0, 0 on the last lane, mneg elsewhere)
-template <int CH, bool REG>
+// (pneg_0 on lane 0, 0 on the last lane, mneg elsewhere).  FULL: n = 64 CH,
+// every lane holds cells.
+template <int CH, bool REG, bool FULL>
 __device__ __forceinline__ void fast_line_chunked(const SweepArgs &a, double *cs,
                                                   const LineK &lk, int lane,
                                                   const LaneTab<CH, REG> &tb, double mfl,
                                                   double mbl, double pf, double pl)
 {
     static_assert(CH >= 8, "chunk-aligned lanes");
-    const int n = a.n, NL = n / CH, NC = n >> 3;
-    const bool full = NL == 64;
-    const bool active = lane < NL;
+    const int n = FULL ? 64 * CH : a.n, NL = FULL ? 64 : n / CH, NC = n >> 3;
+    constexpr bool full = FULL;
+    const bool active = FULL || lane < NL;
     const int base = lane * CH;
 
     // ghost chunks: cells [0, FAST_GHOST) again after cell n-1
@@ -265,7 +266,7 @@ __device__ __forceinline__ void fast_line_chunked(const SweepArgs &a, double *cs
 
     // compact operator, periodic (T2S4_Parareal.cu:616-654 with wrap-around)
     double bprev, bnext;
-    if (full) {
+    if constexpr (full) {
         bprev = dpp_d<DPP_WAVE_ROR1>(b[CH - 1]);
         bnext = dpp_d<DPP_WAVE_ROL1>(b[0]);
     } else {
@@ -312,7 +313,7 @@ __device__ __forceinline__ void fast_line_chunked(const SweepArgs &a, double *cs
         }
     }
     double carry;
-    if (full) {
+    if constexpr (full) {
         carry = dpp_d<DPP_WAVE_SHR1>(Q);
     } else {
         carry = __shfl_up(Q, 1);
@@ -350,7 +351,7 @@ __device__ __forceinline__ void fast_line_chunked(const SweepArgs &a, double *cs
             }
         }
     }
-    if (full) {
+    if constexpr (full) {
         carry = dpp_d<DPP_WAVE_SHL1>(Q);
     } else {
         carry = __shfl_down(Q, 1);
@@ -368,7 +369,7 @@ __device__ __forceinline__ void fast_line_chunked(const SweepArgs &a, double *cs
     // Sherman-Morrison: x = y - ((y_0 + vl y_{n-1}) / (1 + v.z)) z
     const double y0 = readlane_d(r[0], 0);
     const double yn = readlane_d(r[CH - 1], NL - 1);
-    const double fac = (y0 + a.vl * yn) / a.smden;
+    const double fac = (y0 + a.vl * yn) * a.smrcp;
     if (active) {
 #pragma unroll
         for (int k = 0; k < CH; k++) cs[pidx(base + k)] = r[k] - fac * tb.Z(k, base + k);
@@ -393,11 +394,22 @@ __device__ __forceinline__ void sweep_lines(const SweepArgs &a, double *lds, int
             pf = lane == 0 ? pneg0 : tb.M(0, lane * CH);
             pl = lane == NL - 1 ? 0.0 : tb.M(CH - 1, lane * CH + CH - 1);
         }
-        for (int l = wave; l < W; l += NW) {
-            LineK nxt = cur;
-            if (l + NW < W) nxt = line_k(a, L0 + l + NW);
-            fast_line_chunked<CH, REG>(a, lds + l * LP, cur, lane, tb, mfl, mbl, pf, pl);
-            cur = nxt;
+        if (a.n == 64 * CH) {
+            for (int l = wave; l < W; l += NW) {
+                LineK nxt = cur;
+                if (l + NW < W) nxt = line_k(a, L0 + l + NW);
+                fast_line_chunked<CH, REG, true>(a, lds + l * LP, cur, lane, tb, mfl, mbl, pf,
+                                                 pl);
+                cur = nxt;
+            }
+        } else {
+            for (int l = wave; l < W; l += NW) {
+                LineK nxt = cur;
+                if (l + NW < W) nxt = line_k(a, L0 + l + NW);
+                fast_line_chunked<CH, REG, false>(a, lds + l * LP, cur, lane, tb, mfl, mbl, pf,
+                                                  pl);
+                cur = nxt;
+            }
         }
     } else {
         for (int l = wave; l < W; l += NW) {
