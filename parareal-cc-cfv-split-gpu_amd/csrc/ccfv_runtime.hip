@@ -981,11 +981,24 @@ int ccfv_autotune(ccfv_ctx *c, int32_t batch_fields, double h, int32_t out[8])
         for (int b = 0; b < 2 && !status; b++) {
             const int nf = b ? batch_fields : 1;
             const int reps = b ? 3 : 20;
-            SweepArgs a = make_args(c, dir, dir == DIR_X ? 1 : 2,
+            // x: the launch the solves issue, the last x sub-step of a step
+            // fused with the first of the next (one sub-step when fusion is off)
+            const bool fused = dir == DIR_X && c->fuse;
+            SweepArgs a = make_args(c, dir, dir == DIR_X ? (fused ? 3 : 1) : 2,
                                     dir == DIR_X ? *tx : *ty, h, 0.0, 0.0, 0, 0.0);
             a.in = scratch;
             a.out = scratch;
             a.fstride = (long long)N;
+            if (fused) {
+                a.out2 = scratch;
+                a.field0_2 = 0;
+                a.t0_2 = 0.0;
+                a.jdt2 = h;
+            }
+            auto sweep = [&](Tile t) {
+                return fused ? run_sweep_fused(c, post, EPI_NONE, a, nf, t)
+                             : run_sweep(c, dir, false, EPI_NONE, a, nf, t);
+            };
             float best = 1e30f;
             Tile bt = c->tile[d][b];
             for (int w : ws)
@@ -996,10 +1009,9 @@ int ccfv_autotune(ccfv_ctx *c, int32_t batch_fields, double h, int32_t out[8])
                         continue;
                     float tbest = 1e30f;
                     for (int trial = 0; trial < 2; trial++) {
-                        hipError_t e = run_sweep(c, dir, false, EPI_NONE, a, nf, t);
+                        hipError_t e = sweep(t);
                         if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
-                        for (int r = 0; r < reps && e == hipSuccess; r++)
-                            e = run_sweep(c, dir, false, EPI_NONE, a, nf, t);
+                        for (int r = 0; r < reps && e == hipSuccess; r++) e = sweep(t);
                         if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
                         if (e == hipSuccess) e = hipEventSynchronize(e1);
                         float ms = 0.f;
