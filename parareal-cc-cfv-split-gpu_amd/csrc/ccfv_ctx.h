@@ -31,6 +31,10 @@ struct ccfv_ctx {
     // overlapped schedule: the odd iterations' work field and partials, and
     // a copy of U[ns] per iteration parity
     double *Wk2 = nullptr, *partial2 = nullptr, *res2 = nullptr;
+    // x launches of the coarse chain (Parareal epilogues): wider tiles than
+    // the one-field sweeps, so that fewer workgroups share the chip with the
+    // concurrent fine batches; chain_blocks = ny / chain_tile.w
+    ccfv::Tile chain_tile{0, 0};
     int chain_blocks = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // pipelined Parareal: fine propagators of iteration k+1 overlap the
@@ -71,6 +75,8 @@ int hip_fail(hipError_t e, const char *what);
     } while (0)
 
 bool tile_ok(Tile t, int n, int nl, Dir dir, bool post, bool src = false);
+// launch shape of the coarse chain's x launches (ccfv_ctx::chain_tile)
+Tile chain_tile_for(const ccfv_problem &p, Tile fallback);
 double cell_width(const ccfv_problem &P, Dir dir);
 int get_table(ccfv_ctx *c, Dir dir, double h, const SolveTable **out);
 SweepArgs make_args(ccfv_ctx *c, Dir dir, int substep, const SolveTable &T,

@@ -28,6 +28,7 @@ void shared_tiles(ccfv_ctx *c)
     c->tile[DIR_X][1] = xb;
     c->tile[DIR_Y][0] = y1;
     c->tile[DIR_Y][1] = yb;
+    c->chain_tile = chain_tile_for(c->P, x1);
     c->tiles_version++;
     clear_graphs(c);
 }
@@ -239,7 +240,7 @@ StageItem stage_item(ccfv_ctx *c, Dir dir, int substep, const SolveTable &T,
     s.post = dir == DIR_X && substep == 3 && post_flag(c);
     s.epi = substep == 3 ? d.epi : EPI_NONE;
     s.fuse = false;
-    s.tile = c->tile[dir][shape];
+    s.tile = dir == DIR_X && s.epi != EPI_NONE ? c->chain_tile : c->tile[dir][shape];
     SweepArgs &a = s.item.a;
     a = make_args(c, dir, substep, T, d.h, d.t0, d.tDT, d.field0, d.jdt);
     a.in = substep == 1 ? d.in : d.out;

@@ -125,7 +125,7 @@ int place(ccfv_server *s, Request &q)
     c.G = take(f * ns);
     c.F = take(f * ns);
     c.Wk = take(f);
-    c.chain_blocks = c.P.ny / c.tile[DIR_X][0].w;
+    c.chain_blocks = c.P.ny / c.chain_tile.w;
     c.partial = take(sizeof(double) * 2 * (size_t)c.chain_blocks * (ns + 1));
     c.para_ns = ns;
     c.state = nullptr;
