@@ -65,13 +65,14 @@ def parse():
                     help="short run for counter collection: one 32 MiB "
                          "calibration copy and one Parareal iteration")
     ap.add_argument("--workload", default="rotation",
-                    choices=("rotation", "ragged", "serve", "swirl"),
+                    choices=("rotation", "ragged", "serve", "swirl", "copy"),
                     help="rotation: the Parareal benchmark (default); ragged: "
                          "256 independent solves in one batch; serve: Poisson "
                          "request streams through the serving engine; swirl: "
                          "2048^2 deformational swirl, 128 slices, coarse step "
                          "32 fine steps, each Parareal phase one hipGraph, "
-                         "autotuned tiles")
+                         "autotuned tiles; copy: the device copy rate of the "
+                         "library's copy kernel (the practical HBM ceiling)")
     ap.add_argument("--count", type=int, default=256,
                     help="ragged: problems; serve: requests per rate")
     ap.add_argument("--rates", default="50,200,500,1000,2000",
@@ -247,6 +248,28 @@ def serve(a):
     print(json.dumps(out))
 
 
+def copy_rate(a):
+    """Copies for the practical HBM ceiling: the library's copy kernel
+    (ccfv_get_state_device) from one 2048^2 field (32 MiB) to the next, round
+    a ring of 8 fields (256 MiB), 16 B moved per double.  Each call
+    synchronizes, so the rate comes from the kernel durations under
+    rocprofv3 --kernel-trace, not from a host clock."""
+    import ccfv
+    from ccfv import types as T
+    big = T.rotation_problem(2048)
+    ring = [ccfv.Context(big) for _ in range(8)]
+    for c in ring:
+        c.fill_exact(T.rotation_ic(), 0.0)
+        c.sync()
+    ptrs = [c.state_ptr() for c in ring]
+    copies = 8 * max(1, a.steps)
+    for i in range(8 * max(1, a.warmup) + copies):
+        ring[i % 8].get_state_device(ptrs[(i + 1) % 8])
+    print(json.dumps({"workload": "copy", "copies": copies + 8 * max(1, a.warmup),
+                      "bytes_per_copy": 16 * 2048 * 2048,
+                      "kernel": "ccfv::copy_kernel"}))
+
+
 def swirl(a):
     """Large Parareal: 2048^2 swirl u = sin^2(pi x) sin(2 pi y) cos(pi t/5),
     v = -sin^2(pi y) sin(2 pi x) cos(pi t/5), nu = 1e-4, cosine bell of
@@ -317,6 +340,8 @@ def main():
         return serve(a)
     if a.workload == "swirl":
         return swirl(a)
+    if a.workload == "copy":
+        return copy_rate(a)
     import ccfv
     from ccfv import types as T
 
