@@ -392,7 +392,7 @@ __device__ __forceinline__ double src_rhs(const SweepArgs &a, const Times &tt,
     return a.src_w * (fbar + src_avg<DIR>(a, q, line, t1));
 }
 
-template <int DIR, int BC>
+template <int DIR, int BC, bool SRC>
 __device__ void rhs_general(const SweepArgs &a, const Times &tt, const double *cs,
                             const double g[4], double *aux, int base, int ch,
                             const LineGeo &lg, double inv12d, bool post,
@@ -404,7 +404,7 @@ __device__ void rhs_general(const SweepArgs &a, const Times &tt, const double *c
                    : eval_face<DIR, BC, false>(a, tt, cs, g, base, lg, inv12d);
     // with the third-sub-step quirk the source joins after the derivative
     // of the remapped line (the caller adds it)
-    const bool src = fs && !post;
+    const bool src = SRC && fs && !post;
     Face gl;
     if (src) gl = eval_face<DIR, BC, true>(a, tt, fs, fg, base, lg, inv12d);
 #pragma unroll 1

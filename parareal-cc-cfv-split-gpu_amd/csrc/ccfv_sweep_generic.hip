@@ -6,15 +6,88 @@
 // 825-870, :873-917 or :920-963) for a batch of fields.  A block stages W
 // lines in LDS; each wavefront sweeps whole lines; the tile is written back
 // in place (every sweep is line-local).
+//
+// The file is compiled once per chunk size (cells per lane), with
+// -DCCFV_GEN_CH=1, 2, 4, 8, 16 or 32: each object holds that chunk size's
+// kernels and its launchers launch_sweep_generic_chN / launch_ragged_generic_chN.
+// Compiled without CCFV_GEN_CH it holds only the dispatch over chunk sizes.
 #include "ccfv_line.h"
 
 namespace ccfv {
 
+#define CCFV_CAT2(a, b) a##b
+#define CCFV_CAT(a, b) CCFV_CAT2(a, b)
+
+// launchers of one chunk size (defined by the object built for it)
+#define CCFV_GEN_DECL(CHV)                                                              \
+    hipError_t launch_sweep_generic_ch##CHV(Dir dir, int bc, bool post, Epi epi,        \
+                                            const SweepArgs &a, int nfields, Tile tile,  \
+                                            hipStream_t s, bool fuse);                  \
+    hipError_t launch_ragged_generic_ch##CHV(Dir dir, int bc, bool post, Epi epi,        \
+                                             bool fuse, const RaggedItem *items,         \
+                                             const int *prefix, int nitems, int nblocks, \
+                                             Tile t, size_t lds, hipStream_t s);
+CCFV_GEN_DECL(1)
+CCFV_GEN_DECL(2)
+CCFV_GEN_DECL(4)
+CCFV_GEN_DECL(8)
+CCFV_GEN_DECL(16)
+CCFV_GEN_DECL(32)
+
+#ifndef CCFV_GEN_CH
+
+size_t sweep_lds_bytes_generic(int n, Tile tile, bool src)
+{
+    // W tile lines + one auxiliary line per wave (+ one source line per wave)
+    return sizeof(double) * (size_t)line_pitch(n) *
+           (size_t)(tile.w + tile.nw * (src ? 2 : 1));
+}
+
+hipError_t launch_sweep_generic(Dir dir, int bc, bool post, Epi epi,
+                                const SweepArgs &a, int nfields, Tile tile,
+                                hipStream_t s, bool fuse)
+{
+    switch (chunk_for(a.n)) {
+    case 1: return launch_sweep_generic_ch1(dir, bc, post, epi, a, nfields, tile, s, fuse);
+    case 2: return launch_sweep_generic_ch2(dir, bc, post, epi, a, nfields, tile, s, fuse);
+    case 4: return launch_sweep_generic_ch4(dir, bc, post, epi, a, nfields, tile, s, fuse);
+    case 8: return launch_sweep_generic_ch8(dir, bc, post, epi, a, nfields, tile, s, fuse);
+    case 16: return launch_sweep_generic_ch16(dir, bc, post, epi, a, nfields, tile, s, fuse);
+    case 32: return launch_sweep_generic_ch32(dir, bc, post, epi, a, nfields, tile, s, fuse);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_ragged_generic(Dir dir, int bc, bool post, Epi epi, int ch, bool fuse,
+                                 const RaggedItem *items, const int *prefix,
+                                 int nitems, int nblocks, Tile t, size_t lds,
+                                 hipStream_t s)
+{
+    if (nblocks <= 0 || nitems <= 0) return hipSuccess;
+    switch (ch) {
+#define CCFV_RG(CHV)                                                                        \
+    case CHV:                                                                               \
+        return launch_ragged_generic_ch##CHV(dir, bc, post, epi, fuse, items, prefix, nitems, \
+                                             nblocks, t, lds, s);
+        CCFV_RG(1)
+        CCFV_RG(2)
+        CCFV_RG(4)
+        CCFV_RG(8)
+        CCFV_RG(16)
+        CCFV_RG(32)
+#undef CCFV_RG
+    }
+    return hipErrorInvalidValue;
+}
+
+#else  // CCFV_GEN_CH: the kernels of one chunk size
+
 // cs: the line in LDS (pidx layout), overwritten with the result.
 // aux: a line-sized LDS buffer owned by this wave.
 // fs: a second line-sized LDS buffer of this wave, for the source's earlier
-// cell averages (st: the source's times; used when a.src_kind != 0).
-template <int DIR, int BC, int CH, bool POST>
+// cell averages (st: the source's times).  SRC: the problem has a source
+// (a.src_kind != 0); the source-free instance carries none of its code.
+template <int DIR, int BC, int CH, bool POST, bool SRC>
 __device__ void process_line(const SweepArgs &a, const Times tt, double *cs, double *aux,
                              double *fs, const SrcTimes st, int line, int lane)
 {
@@ -26,7 +99,7 @@ __device__ void process_line(const SweepArgs &a, const Times tt, double *cs, dou
     const LineGeo lg = line_geo(a, tt, line);
     double g[4] = {0.0, 0.0, 0.0, 0.0};
     if (BC == BC_REF) ghosts(cs, n, g);
-    const bool src = a.src_kind != CCFV_SRC_NONE;
+    constexpr bool src = SRC;
     double fg[4] = {0.0, 0.0, 0.0, 0.0};
     if (src) {
         // the source's earlier cell averages of this line (T2S4_Parareal.cu:
@@ -39,7 +112,7 @@ __device__ void process_line(const SweepArgs &a, const Times tt, double *cs, dou
 
     double b[CH];
     if (active)
-        rhs_general<DIR, BC>(a, tt, cs, g, aux, base, CH, lg, inv12d, POST,
+        rhs_general<DIR, BC, SRC>(a, tt, cs, g, aux, base, CH, lg, inv12d, POST,
                              src ? fs : nullptr, fg, line, st.t1);
     if (POST) {
         // the reference's third sub-step takes the derivative of the remapped
@@ -88,7 +161,7 @@ __device__ void process_line(const SweepArgs &a, const Times tt, double *cs, dou
 // FUSE: after the sweep (and its epilogue, whose result stays in LDS) the
 // same lines take the next step's first x sub-step (times from field0_2 and
 // jdt2, the same step size) and go to out2.
-template <int BC, int CH, bool POST, int EPI, bool FUSE>
+template <int BC, int CH, bool POST, int EPI, bool FUSE, bool SRC>
 __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, int field, int tile,
                                              int W, int NW, double *lds)
 {
@@ -103,7 +176,7 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, int field, int 
     const Times tt = sub_times(a, field);
     const SrcTimes st = src_times(a, field, false);
     for (int l = wave; l < W; l += NW)
-        process_line<DIR_X, BC, CH, POST>(a, tt, lds + l * LP, aux, fs, st, j0 + l, lane);
+        process_line<DIR_X, BC, CH, POST, SRC>(a, tt, lds + l * LP, aux, fs, st, j0 + l, lane);
     __syncthreads();
     if (!FUSE) {
         if (EPI == EPI_NONE) store_tile_x(a.out + foff, a.ld, n, j0, W, lds, LP);
@@ -117,12 +190,13 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, int field, int 
     const Times t2 = sub_times_next(a, field);
     const SrcTimes st2 = src_times(a, field, true);
     for (int l = wave; l < W; l += NW)
-        process_line<DIR_X, BC, CH, false>(a, t2, lds + l * LP, aux, fs, st2, j0 + l, lane);
+        process_line<DIR_X, BC, CH, false, SRC>(a, t2, lds + l * LP, aux, fs, st2, j0 + l,
+                                                lane);
     __syncthreads();
     store_tile_x(a.out2 + foff, a.ld, n, j0, W, lds, LP);
 }
 
-template <int BC, int CH>
+template <int BC, int CH, bool SRC>
 __device__ __forceinline__ void sweep_y_body(const SweepArgs &a, int field, int tile,
                                              int W, int NW, double *lds)
 {
@@ -137,24 +211,26 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs &a, int field, int 
     const Times tt = sub_times(a, field);
     const SrcTimes st = src_times(a, field, false);
     for (int l = wave; l < W; l += NW)
-        process_line<DIR_Y, BC, CH, false>(a, tt, lds + l * LP, aux, fs, st, i0 + l, lane);
+        process_line<DIR_Y, BC, CH, false, SRC>(a, tt, lds + l * LP, aux, fs, st, i0 + l, lane);
     __syncthreads();
     store_tile_y(a.out + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
 }
 
-template <int BC, int CH, bool POST, int EPI, bool FUSE>
+// SRC: problems with a source term take their own instances, so the
+// source-free kernels carry none of its code or registers
+template <int BC, int CH, bool POST, int EPI, bool FUSE, bool SRC>
 __global__ __launch_bounds__(512) void sweep_x_kernel(SweepArgs a, int W, int NW)
 {
     extern __shared__ double lds[];
-    sweep_x_body<BC, CH, POST, EPI, FUSE>(a, blockIdx.y, xcd_tile(blockIdx.x, gridDim.x),
-                                          W, NW, lds);
+    sweep_x_body<BC, CH, POST, EPI, FUSE, SRC>(a, blockIdx.y,
+                                               xcd_tile(blockIdx.x, gridDim.x), W, NW, lds);
 }
 
-template <int BC, int CH>
+template <int BC, int CH, bool SRC>
 __global__ __launch_bounds__(512) void sweep_y_kernel(SweepArgs a, int W, int NW)
 {
     extern __shared__ double lds[];
-    sweep_y_body<BC, CH>(a, blockIdx.y, blockIdx.x, W, NW, lds);
+    sweep_y_body<BC, CH, SRC>(a, blockIdx.y, blockIdx.x, W, NW, lds);
 }
 
 template <int BC, int CH, bool POST, int EPI, bool FUSE>
@@ -171,7 +247,11 @@ __global__ __launch_bounds__(512) void ragged_x_kernel(const RaggedItem *items,
     // the arguments stay in global memory (scalar-cached); a private copy of
     // the struct trips the gfx950 backend for this kernel
     const SweepArgs &a = items[i].a;
-    sweep_x_body<BC, CH, POST, EPI, FUSE>(a, field, local - field * nt, W, NW, lds);
+    // the items of a ragged launch may mix problems with and without a source
+    if (a.src_kind != CCFV_SRC_NONE)
+        sweep_x_body<BC, CH, POST, EPI, FUSE, true>(a, field, local - field * nt, W, NW, lds);
+    else
+        sweep_x_body<BC, CH, POST, EPI, FUSE, false>(a, field, local - field * nt, W, NW, lds);
 }
 
 template <int BC, int CH>
@@ -186,14 +266,10 @@ __global__ __launch_bounds__(512) void ragged_y_kernel(const RaggedItem *items,
     const int nt = items[i].ntiles;
     const int field = local / nt;
     const SweepArgs &a = items[i].a;
-    sweep_y_body<BC, CH>(a, field, local - field * nt, W, NW, lds);
-}
-
-size_t sweep_lds_bytes_generic(int n, Tile tile, bool src)
-{
-    // W tile lines + one auxiliary line per wave (+ one source line per wave)
-    return sizeof(double) * (size_t)line_pitch(n) *
-           (size_t)(tile.w + tile.nw * (src ? 2 : 1));
+    if (a.src_kind != CCFV_SRC_NONE)
+        sweep_y_body<BC, CH, true>(a, field, local - field * nt, W, NW, lds);
+    else
+        sweep_y_body<BC, CH, false>(a, field, local - field * nt, W, NW, lds);
 }
 
 template <typename K>
@@ -211,56 +287,50 @@ static hipError_t launch(K k, dim3 grid, dim3 block, size_t lds, hipStream_t s,
     return hipGetLastError();
 }
 
-template <int BC, int CH, bool POST, bool FUSE>
+template <int BC, int CH, bool POST, bool FUSE, bool SRC>
 static hipError_t dispatch_x(Epi epi, dim3 grid, dim3 block, size_t lds, hipStream_t s,
                              const SweepArgs &a, Tile t)
 {
     if (epi == EPI_NONE)
-        return launch(sweep_x_kernel<BC, CH, POST, EPI_NONE, FUSE>, grid, block, lds, s, a,
-                      t);
+        return launch(sweep_x_kernel<BC, CH, POST, EPI_NONE, FUSE, SRC>, grid, block, lds, s,
+                      a, t);
     if (epi == EPI_INIT)
-        return launch(sweep_x_kernel<BC, CH, POST, EPI_INIT, FUSE>, grid, block, lds, s, a,
-                      t);
-    return launch(sweep_x_kernel<BC, CH, POST, EPI_CORRECT, FUSE>, grid, block, lds, s, a,
-                  t);
+        return launch(sweep_x_kernel<BC, CH, POST, EPI_INIT, FUSE, SRC>, grid, block, lds, s,
+                      a, t);
+    return launch(sweep_x_kernel<BC, CH, POST, EPI_CORRECT, FUSE, SRC>, grid, block, lds, s,
+                  a, t);
+}
+
+template <int BC, int CH, bool SRC>
+static hipError_t dispatch_src(Dir dir, bool post, Epi epi, bool fuse, const SweepArgs &a,
+                               int nf, Tile t, hipStream_t s)
+{
+    dim3 grid(a.nl / t.w, nf), block(64 * t.nw);
+    size_t lds = sweep_lds_bytes_generic(a.n, t, SRC);
+    if (dir == DIR_Y) return launch(sweep_y_kernel<BC, CH, SRC>, grid, block, lds, s, a, t);
+    if (BC == BC_REF && post)
+        return fuse ? dispatch_x<BC, CH, true, true, SRC>(epi, grid, block, lds, s, a, t)
+                    : dispatch_x<BC, CH, true, false, SRC>(epi, grid, block, lds, s, a, t);
+    return fuse ? dispatch_x<BC, CH, false, true, SRC>(epi, grid, block, lds, s, a, t)
+                : dispatch_x<BC, CH, false, false, SRC>(epi, grid, block, lds, s, a, t);
 }
 
 template <int BC, int CH>
 static hipError_t dispatch(Dir dir, bool post, Epi epi, bool fuse, const SweepArgs &a,
                            int nf, Tile t, hipStream_t s)
 {
-    dim3 grid(a.nl / t.w, nf), block(64 * t.nw);
-    size_t lds = sweep_lds_bytes_generic(a.n, t, a.src_kind != CCFV_SRC_NONE);
-    if (dir == DIR_Y) return launch(sweep_y_kernel<BC, CH>, grid, block, lds, s, a, t);
-    if (BC == BC_REF && post)
-        return fuse ? dispatch_x<BC, CH, true, true>(epi, grid, block, lds, s, a, t)
-                    : dispatch_x<BC, CH, true, false>(epi, grid, block, lds, s, a, t);
-    return fuse ? dispatch_x<BC, CH, false, true>(epi, grid, block, lds, s, a, t)
-                : dispatch_x<BC, CH, false, false>(epi, grid, block, lds, s, a, t);
+    if (a.src_kind != CCFV_SRC_NONE)
+        return dispatch_src<BC, CH, true>(dir, post, epi, fuse, a, nf, t, s);
+    return dispatch_src<BC, CH, false>(dir, post, epi, fuse, a, nf, t, s);
 }
 
-template <int BC>
-static hipError_t dispatch_bc(Dir dir, bool post, Epi epi, bool fuse, const SweepArgs &a,
-                              int nf, Tile t, hipStream_t s)
-{
-    switch (chunk_for(a.n)) {
-    case 1: return dispatch<BC, 1>(dir, post, epi, fuse, a, nf, t, s);
-    case 2: return dispatch<BC, 2>(dir, post, epi, fuse, a, nf, t, s);
-    case 4: return dispatch<BC, 4>(dir, post, epi, fuse, a, nf, t, s);
-    case 8: return dispatch<BC, 8>(dir, post, epi, fuse, a, nf, t, s);
-    case 16: return dispatch<BC, 16>(dir, post, epi, fuse, a, nf, t, s);
-    case 32: return dispatch<BC, 32>(dir, post, epi, fuse, a, nf, t, s);
-    }
-    return hipErrorInvalidValue;
-}
-
-hipError_t launch_sweep_generic(Dir dir, int bc, bool post, Epi epi,
-                                const SweepArgs &a, int nfields, Tile tile,
-                                hipStream_t s, bool fuse)
+hipError_t CCFV_CAT(launch_sweep_generic_ch, CCFV_GEN_CH)(Dir dir, int bc, bool post, Epi epi,
+                                                        const SweepArgs &a, int nfields,
+                                                        Tile tile, hipStream_t s, bool fuse)
 {
     if (bc == BC_PER)
-        return dispatch_bc<BC_PER>(dir, false, epi, fuse, a, nfields, tile, s);
-    return dispatch_bc<BC_REF>(dir, post, epi, fuse, a, nfields, tile, s);
+        return dispatch<BC_PER, CCFV_GEN_CH>(dir, false, epi, fuse, a, nfields, tile, s);
+    return dispatch<BC_REF, CCFV_GEN_CH>(dir, post, epi, fuse, a, nfields, tile, s);
 }
 
 template <typename K>
@@ -306,33 +376,20 @@ static hipError_t dispatch_r(Dir dir, bool post, Epi epi, bool fuse, const Ragge
                 : dispatch_rx<BC, CH, false, false>(epi, it, pf, ni, nb, t, lds, s);
 }
 
-template <int BC>
-static hipError_t dispatch_rbc(Dir dir, bool post, Epi epi, bool fuse, int ch,
-                               const RaggedItem *it, const int *pf, int ni, int nb, Tile t,
-                               size_t lds, hipStream_t s)
+hipError_t CCFV_CAT(launch_ragged_generic_ch, CCFV_GEN_CH)(Dir dir, int bc, bool post,
+                                                         Epi epi, bool fuse,
+                                                         const RaggedItem *items,
+                                                         const int *prefix, int nitems,
+                                                         int nblocks, Tile t, size_t lds,
+                                                         hipStream_t s)
 {
-    switch (ch) {
-    case 1: return dispatch_r<BC, 1>(dir, post, epi, fuse, it, pf, ni, nb, t, lds, s);
-    case 2: return dispatch_r<BC, 2>(dir, post, epi, fuse, it, pf, ni, nb, t, lds, s);
-    case 4: return dispatch_r<BC, 4>(dir, post, epi, fuse, it, pf, ni, nb, t, lds, s);
-    case 8: return dispatch_r<BC, 8>(dir, post, epi, fuse, it, pf, ni, nb, t, lds, s);
-    case 16: return dispatch_r<BC, 16>(dir, post, epi, fuse, it, pf, ni, nb, t, lds, s);
-    case 32: return dispatch_r<BC, 32>(dir, post, epi, fuse, it, pf, ni, nb, t, lds, s);
-    }
-    return hipErrorInvalidValue;
+    if (bc == BC_PER)
+        return dispatch_r<BC_PER, CCFV_GEN_CH>(dir, false, epi, fuse, items, prefix, nitems,
+                                               nblocks, t, lds, s);
+    return dispatch_r<BC_REF, CCFV_GEN_CH>(dir, post, epi, fuse, items, prefix, nitems,
+                                           nblocks, t, lds, s);
 }
 
-hipError_t launch_ragged_generic(Dir dir, int bc, bool post, Epi epi, int ch, bool fuse,
-                                 const RaggedItem *items, const int *prefix,
-                                 int nitems, int nblocks, Tile t, size_t lds,
-                                 hipStream_t s)
-{
-    if (nblocks <= 0 || nitems <= 0) return hipSuccess;
-    if (bc == BC_PER)
-        return dispatch_rbc<BC_PER>(dir, false, epi, fuse, ch, items, prefix, nitems,
-                                    nblocks, t, lds, s);
-    return dispatch_rbc<BC_REF>(dir, post, epi, fuse, ch, items, prefix, nitems, nblocks,
-                                t, lds, s);
-}
+#endif  // CCFV_GEN_CH
 
 }  // namespace ccfv
