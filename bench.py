@@ -374,6 +374,12 @@ def main():
                      a.warmup)
     plain_ms = timed(ctx, reset, lambda: ctx.plain_solve(0.0, 1.0, a.nt),
                      a.steps)
+    # the same solve captured once as a hipGraph and replayed (identical
+    # results; the graph's launches skip the per-launch submission work)
+    ctx.set_option(T.OPT_PLAIN_GRAPHS, 1)
+    timed(ctx, reset, lambda: ctx.plain_solve(0.0, 1.0, a.nt), a.warmup)
+    plain_graph_ms = timed(ctx, reset, lambda: ctx.plain_solve(0.0, 1.0, a.nt), a.steps)
+    ctx.set_option(T.OPT_PLAIN_GRAPHS, 0)
     reset()
     ctx.plain_solve(0.0, 1.0, a.nt)
     uref = ctx.get_state()
@@ -438,6 +444,7 @@ def main():
         "plain_value": plain_value,
         "plain_p50_ms": pp50,
         "plain_p99_ms": pp99,
+        "plain_graph_p50_ms": pct(plain_graph_ms, 50),
         "speedup_vs_plain": pp50 / p50,
         "parareal_iterations": rep.iterations,
         "parareal_converged": bool(rep.converged),
