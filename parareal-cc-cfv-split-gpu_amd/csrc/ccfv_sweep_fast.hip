@@ -235,11 +235,13 @@ __device__ __forceinline__ void load_window(const double *p, int r, double (&cw)
 // pf / pl: the back-substitution factor of the lane's first / last cell
 // (pneg_0 on lane 0, 0 on the last lane, mneg elsewhere).  FULL: n = 64 CH,
 // every lane holds cells.
+// res: when given, the lane's CH results go there instead of back to cs
 template <int CH, bool REG, bool FULL>
 __device__ __forceinline__ void fast_line_chunked(const SweepArgs &a, double *cs,
                                                   const LineK &lk, int lane,
                                                   const LaneTab<CH, REG> &tb, double mfl,
-                                                  double mbl, double pf, double pl)
+                                                  double mbl, double pf, double pl,
+                                                  double (*res)[CH] = nullptr)
 {
     static_assert(CH >= 8, "chunk-aligned lanes");
     const int n = FULL ? 64 * CH : a.n, NL = FULL ? 64 : n / CH, NC = n >> 3;
@@ -370,7 +372,10 @@ __device__ __forceinline__ void fast_line_chunked(const SweepArgs &a, double *cs
     const double y0 = readlane_d(r[0], 0);
     const double yn = readlane_d(r[CH - 1], NL - 1);
     const double fac = (y0 + a.vl * yn) * a.smrcp;
-    if (active) {
+    if (res) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) (*res)[k] = r[k] - fac * tb.Z(k, base + k);
+    } else if (active) {
 #pragma unroll
         for (int k = 0; k < CH; k++) cs[pidx(base + k)] = r[k] - fac * tb.Z(k, base + k);
     }
@@ -461,10 +466,76 @@ __device__ __forceinline__ void fast_x_body(const SweepArgs &a, int field, int t
     store_tile_x(a.out2 + foff, a.ld, n, j0, W, lds, LP);
 }
 
+// y lines of 64*CH cells (CH >= 8), one wave per line (W == NW, the
+// one-field launches): a y line is a contiguous row, so each lane moves its
+// own CH cells between HBM and registers (CH/2 16-byte loads and stores) and
+// the line passes through LDS only for the shifted windows.  No barriers,
+// and no LDS round trip before the store: the launch's latency drops
+// (one-field y sweep at 512^2 6.7 against 7.6 us).  Batches keep the
+// coalesced tile loads: per-lane chunks touch four times as many cache
+// lines per instruction, which costs more than this saves once the launch
+// is bandwidth-bound (64 fields: 57 against 47 us).
+template <int CH>
+__device__ __forceinline__ void fast_y_direct(const SweepArgs &a, int field, int tile,
+                                             int W, int NW, double *lds)
+{
+    static_assert(CH >= 8, "full-wave lines of whole chunks");
+    constexpr bool REG = CH <= 8;
+    constexpr int n = 64 * CH;
+    const int LP = line_pitch(n + fast_ghost_cells(CH));
+    const size_t foff = (size_t)field * (size_t)a.fstride;
+    const int i0 = tile * W;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    double *cs = lds + wave * LP;
+    LaneTab<CH, REG> tb;
+    tb.load(a, lane * CH, true);
+    const double mfl = a.tab[4 * n + lane], mbl = a.tab[4 * n + 64 + lane];
+    const double pneg0 = a.tab[2 * n];
+    const double pf = lane == 0 ? pneg0 : tb.M(0, lane * CH);
+    const double pl = lane == 63 ? 0.0 : tb.M(CH - 1, lane * CH + CH - 1);
+    const double *in = a.in + foff + lane * CH;
+    double *out = a.out + foff + lane * CH;
+    double2 nxt[CH / 2];
+    int l = wave;
+    if (l < W) {
+        const double2 *src = reinterpret_cast<const double2 *>(in + (size_t)(i0 + l) * a.ld);
+#pragma unroll
+        for (int k = 0; k < CH / 2; k++) nxt[k] = src[k];
+    }
+    for (; l < W; l += NW) {
+        const LineK lk = line_k(a, i0 + l);
+        // the lane's chunk sits at pidx(CH * lane) = 9 CH/8 * lane, contiguous
+        double *own = cs + pidx(CH * lane);
+#pragma unroll
+        for (int k = 0; k < CH / 2; k++) {
+            own[2 * k + (2 * k >> 3)] = nxt[k].x;
+            own[2 * k + 1 + ((2 * k + 1) >> 3)] = nxt[k].y;
+        }
+        if (l + NW < W) {
+            const double2 *src =
+                reinterpret_cast<const double2 *>(in + (size_t)(i0 + l + NW) * a.ld);
+#pragma unroll
+            for (int k = 0; k < CH / 2; k++) nxt[k] = src[k];
+        }
+        double res[CH];
+        fast_line_chunked<CH, REG, true>(a, cs, lk, lane, tb, mfl, mbl, pf, pl, &res);
+        double2 *dst = reinterpret_cast<double2 *>(out + (size_t)(i0 + l) * a.ld);
+#pragma unroll
+        for (int k = 0; k < CH / 2; k++) dst[k] = make_double2(res[2 * k], res[2 * k + 1]);
+    }
+}
+
 template <int CH>
 __device__ __forceinline__ void fast_y_body(const SweepArgs &a, int field, int tile,
                                            int W, int NW, double *lds)
 {
+    if constexpr (CH >= 8) {
+        if (a.n == 64 * CH && W == NW) {
+            fast_y_direct<CH>(a, field, tile, W, NW, lds);
+            return;
+        }
+    }
     constexpr bool REG = CH <= 8;
     const int n = a.n, LP = line_pitch(n + fast_ghost_cells(CH));
     const size_t foff = (size_t)field * (size_t)a.fstride;
