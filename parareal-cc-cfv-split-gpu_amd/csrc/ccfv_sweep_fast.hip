@@ -426,6 +426,52 @@ __device__ __forceinline__ void sweep_lines(const SweepArgs &a, double *lds, int
     }
 }
 
+// x lines of 64*CH cells (CH >= 8), one wave per line (W == NW, the
+// one-field launches), no epilogue: each lane gathers its CH cells of the
+// column into registers, runs the sweep (and, fused, the next step's first
+// x sweep) through the wave's own LDS line, and scatters its results back.
+// No barriers and no LDS round trip before the store.
+template <int CH, bool FUSE>
+__device__ __forceinline__ void fast_x_direct(const SweepArgs &a, int field, int tile,
+                                             int W, double *lds)
+{
+    static_assert(CH >= 8, "full-wave lines of whole chunks");
+    constexpr bool REG = CH <= 8;
+    constexpr int n = 64 * CH;
+    const int LP = line_pitch(n + fast_ghost_cells(CH));
+    const size_t foff = (size_t)field * (size_t)a.fstride;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int j = tile * W + wave;
+    double *cs = lds + wave * LP;
+    const size_t row0 = (size_t)(lane * CH) * a.ld;
+    const double *in = a.in + foff + j + row0;
+    double v[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) v[k] = in[(size_t)k * a.ld];
+    LaneTab<CH, REG> tb;
+    tb.load(a, lane * CH, true);
+    const double mfl = a.tab[4 * n + lane], mbl = a.tab[4 * n + 64 + lane];
+    const double pneg0 = a.tab[2 * n];
+    const double pf = lane == 0 ? pneg0 : tb.M(0, lane * CH);
+    const double pl = lane == 63 ? 0.0 : tb.M(CH - 1, lane * CH + CH - 1);
+    const LineK lk = line_k(a, j);
+    // the lane's chunk sits at pidx(CH * lane), CH cells and CH/8 pads
+    double *own = cs + pidx(CH * lane);
+#pragma unroll
+    for (int k = 0; k < CH; k++) own[k + (k >> 3)] = v[k];
+    double res[CH];
+    if (FUSE) {
+        fast_line_chunked<CH, REG, true>(a, cs, lk, lane, tb, mfl, mbl, pf, pl);
+        fast_line_chunked<CH, REG, true>(a, cs, lk, lane, tb, mfl, mbl, pf, pl, &res);
+    } else {
+        fast_line_chunked<CH, REG, true>(a, cs, lk, lane, tb, mfl, mbl, pf, pl, &res);
+    }
+    double *out = (FUSE ? a.out2 : a.out) + foff + j + row0;
+#pragma unroll
+    for (int k = 0; k < CH; k++) out[(size_t)k * a.ld] = res[k];
+}
+
 // FUSE: after the sweep (and its epilogue, whose result stays in LDS) the
 // same lines take the next step's first x sub-step, with the same table
 // (one step size) and the same per-line shift, and go to out2.
@@ -433,6 +479,12 @@ template <int CH, int EPI, bool FUSE>
 __device__ __forceinline__ void fast_x_body(const SweepArgs &a, int field, int tile,
                                            int W, int NW, double *lds)
 {
+    if constexpr (CH >= 8 && EPI == EPI_NONE) {
+        if (a.n == 64 * CH && W == NW) {
+            fast_x_direct<CH, FUSE>(a, field, tile, W, lds);
+            return;
+        }
+    }
     constexpr bool REG = CH <= 8;
     const int n = a.n, LP = line_pitch(n + fast_ghost_cells(CH));
     const size_t foff = (size_t)field * (size_t)a.fstride;
