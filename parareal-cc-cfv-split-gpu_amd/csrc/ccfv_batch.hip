@@ -108,6 +108,7 @@ int ccfv_batch_context(ccfv_batch *b, int32_t i, ccfv_ctx **ctx)
 int ccfv_batch_plain(ccfv_batch *b, const double *t0, const double *t1,
                      const int32_t *nt)
 {
+    ccfv::rt::TraceRange tr("ccfv_batch_plain");
     if (!b || !t0 || !t1 || !nt) return fail(CCFV_EINVAL, "NULL argument");
     const int np = (int)b->ctx.size();
     int rc;
@@ -149,6 +150,7 @@ int ccfv_batch_parareal(ccfv_batch *b, const double *t0, const double *t1,
                         const int32_t *nt_fine, const int32_t *n_slices, int32_t kmax,
                         double tol, ccfv_parareal_report *reports)
 {
+    ccfv::rt::TraceRange tr("ccfv_batch_parareal");
     if (!b || !t0 || !t1 || !nt_fine || !n_slices)
         return fail(CCFV_EINVAL, "NULL argument");
     if (kmax < 0 || kmax > CCFV_MAX_ITERS) return fail(CCFV_EINVAL, "kmax must be in [0, 64]");

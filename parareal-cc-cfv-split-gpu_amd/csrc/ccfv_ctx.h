@@ -8,6 +8,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "ccfv_internal.h"
 
 struct ccfv_ctx {
@@ -75,6 +77,15 @@ int hip_fail(hipError_t e, const char *what);
     } while (0)
 
 bool tile_ok(Tile t, int n, int nl, Dir dir, bool post, bool src = false);
+
+// Host range for rocprofv3 --marker-trace (a solve, a Parareal iteration, a
+// batch or serving step); without a tool attached the calls return at once.
+struct TraceRange {
+    explicit TraceRange(const char *name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange &) = delete;
+    TraceRange &operator=(const TraceRange &) = delete;
+};
 // launch shape of the coarse chain's x launches (ccfv_ctx::chain_tile)
 Tile chain_tile_for(const ccfv_problem &p, Tile fallback);
 double cell_width(const ccfv_problem &P, Dir dir);

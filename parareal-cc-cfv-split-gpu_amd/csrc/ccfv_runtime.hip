@@ -710,6 +710,10 @@ int ccfv_create(const ccfv_problem *p, ccfv_ctx **out)
     }
     *c->h_flag = 0;
     c->cur = c->stream;
+    // stream names in rocprofv3 traces
+    roctxNameHipStream("ccfv chain", c->stream);
+    roctxNameHipStream("ccfv fine", c->stream2);
+    roctxNameHipStream("ccfv chain (odd iterations)", c->stream3);
     if (const char *s = getenv("CCFV_PIPE")) c->pipeline = std::min(2, std::max(0, atoi(s)));
     if (const char *s = getenv("CCFV_PIPE_GROUP")) c->pipe_group = std::max(1, atoi(s));
     if (const char *s = getenv("CCFV_GRAPHS")) c->graphs = atoi(s) != 0;
@@ -855,6 +859,7 @@ int ccfv_advance(ccfv_ctx *c, double t_origin, double dt, int32_t n_begin, int32
     if (n_begin < 0 || n_end < n_begin) return fail(CCFV_EINVAL, "need 0 <= n_begin <= n_end");
     if (!(dt > 0.0) || !std::isfinite(dt) || !std::isfinite(t_origin))
         return fail(CCFV_EINVAL, "step size must be finite and > 0");
+    ccfv::rt::TraceRange tr("ccfv_advance");
     c->cur = c->stream;
     int rc = run_steps(c, n_end - n_begin, [&](int i) {
         const int n = n_begin + i;
@@ -872,6 +877,7 @@ int ccfv_plain_solve(ccfv_ctx *c, double t0, double t1, int32_t nt)
     if (!(t1 > t0) || !std::isfinite(t0) || !std::isfinite(t1))
         return fail(CCFV_EINVAL, "need finite t1 > t0");
     double dt = (t1 - t0) / nt;
+    ccfv::rt::TraceRange tr("ccfv_plain_solve");
     c->cur = c->stream;
     auto body = [&]() -> int {
         return run_steps(c, nt, [&](int n) {
@@ -1155,6 +1161,9 @@ int parareal_sequential(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R
     R.coarse_steps_run = ns;
     int k;
     for (k = 0; k < pl.kmax; k++) {
+        char label[48];
+        snprintf(label, sizeof(label), "parareal iteration %d", k);
+        TraceRange tr(label);
         const int nf = ns - k;
         if ((rc = run_steps(c, J, [&](int j) { return fine_step(c, pl, k, nf, j); })))
             return rc;
@@ -1357,6 +1366,9 @@ int parareal_pipelined(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R,
     int k;
     bool pending = false;  // increments still to be read (tol <= 0)
     for (k = 0; k < pl.kmax; k++) {
+        char label[48];
+        snprintf(label, sizeof(label), "parareal iteration %d", k);
+        TraceRange tr(label);
         rc = phase(k, [&]() -> int {
             int r2 = enqueue_chain(k);
             if (r2 == CCFV_OK && k + 1 < pl.kmax) r2 = enqueue_fine(k + 1);
@@ -1603,6 +1615,7 @@ int ccfv_parareal_solve(ccfv_ctx *c, double t0, double t1, int32_t nt_fine,
     if (kmax < 0 || kmax > CCFV_MAX_ITERS)
         return fail(CCFV_EINVAL, "kmax must be in [0, 64]");
     if (!(t1 > t0)) return fail(CCFV_EINVAL, "need t1 > t0");
+    ccfv::rt::TraceRange tr("ccfv_parareal_solve");
     int rc = ensure_para(c, ns);
     if (rc) return rc;
     const size_t N = c->ncell;

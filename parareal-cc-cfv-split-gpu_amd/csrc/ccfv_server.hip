@@ -278,6 +278,7 @@ int ccfv_server_submit(ccfv_server *s, const ccfv_request *r, const double *u0, 
 
 int ccfv_server_step(ccfv_server *s, ccfv_server_status *st)
 {
+    ccfv::rt::TraceRange tr("ccfv_server_step");
     if (!s) return fail(CCFV_EINVAL, "NULL server");
     int rc;
     const hipStream_t stream = s->run.stream;
