@@ -475,12 +475,15 @@ __device__ __forceinline__ void fast_x_direct(const SweepArgs &a, int field, int
 // FUSE: after the sweep (and its epilogue, whose result stays in LDS) the
 // same lines take the next step's first x sub-step, with the same table
 // (one step size) and the same per-line shift, and go to out2.
+// direct: the launch sweeps one field alone (not a batch, not a ragged
+// launch over many problems), where the per-lane direct access pays; a
+// bandwidth-bound launch of many fields keeps the coalesced tile loads
 template <int CH, int EPI, bool FUSE>
 __device__ __forceinline__ void fast_x_body(const SweepArgs &a, int field, int tile,
-                                           int W, int NW, double *lds)
+                                           int W, int NW, double *lds, bool direct = false)
 {
     if constexpr (CH >= 8 && EPI == EPI_NONE) {
-        if (a.n == 64 * CH && W == NW) {
+        if (direct && a.n == 64 * CH && W == NW) {
             fast_x_direct<CH, FUSE>(a, field, tile, W, lds);
             return;
         }
@@ -580,10 +583,10 @@ __device__ __forceinline__ void fast_y_direct(const SweepArgs &a, int field, int
 
 template <int CH>
 __device__ __forceinline__ void fast_y_body(const SweepArgs &a, int field, int tile,
-                                           int W, int NW, double *lds)
+                                           int W, int NW, double *lds, bool direct = false)
 {
     if constexpr (CH >= 8) {
-        if (a.n == 64 * CH && W == NW) {
+        if (direct && a.n == 64 * CH && W == NW) {
             fast_y_direct<CH>(a, field, tile, W, NW, lds);
             return;
         }
@@ -611,14 +614,14 @@ __global__ __launch_bounds__(256) void fast_x_kernel(SweepArgs a, int W, int NW)
 {
     extern __shared__ double lds[];
     fast_x_body<CH, EPI, FUSE>(a, blockIdx.y, xcd_tile(blockIdx.x, gridDim.x), W, NW,
-                               lds);
+                               lds, gridDim.y == 1);
 }
 
 template <int CH>
 __global__ __launch_bounds__(256) void fast_y_kernel(SweepArgs a, int W, int NW)
 {
     extern __shared__ double lds[];
-    fast_y_body<CH>(a, blockIdx.y, blockIdx.x, W, NW, lds);
+    fast_y_body<CH>(a, blockIdx.y, blockIdx.x, W, NW, lds, gridDim.y == 1);
 }
 
 // ragged: one launch over many problems of the same line class
