@@ -506,7 +506,9 @@ __device__ __forceinline__ void fast_x_body(const SweepArgs &a, int field, int t
     load_tile_x(a.in + foff, a.ld, n, j0, W, lds, LP);
     __syncthreads();
     sweep_lines<CH, REG>(a, lds, LP, j0, W, NW, wave, lane, first, tb, mfl, mbl, pneg0);
-    __syncthreads();
+    // the second sweep of a fused launch without an epilogue touches only the
+    // wave's own lines: no barrier before it
+    if (!FUSE || EPI != EPI_NONE) __syncthreads();
     if (!FUSE) {
         if (EPI == EPI_NONE) store_tile_x(a.out + foff, a.ld, n, j0, W, lds, LP);
         else epilogue_x<EPI>(a, n, j0, W, lds, LP, tile, &pre);
