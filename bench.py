@@ -107,21 +107,28 @@ def cpu_baseline(p, ic, nt, budget_s):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle  # noqa: E402  (CPU restatement: baseline leg only)
     threads = min(16, os.cpu_count() or 1)
-    pyoracle.set_threads(threads)
-    u = pyoracle.exact(p, ic, 0.0)
-    dt = 1.0 / nt
-    steps = 0
-    t0 = time.perf_counter()
-    while steps < nt:
-        u = pyoracle.step(p, u, steps * dt, dt)
-        steps += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    el = time.perf_counter() - t0
+
+    def run(nthreads, budget):
+        pyoracle.set_threads(nthreads)
+        u = pyoracle.exact(p, ic, 0.0)
+        dt = 1.0 / nt
+        steps = 0
+        t0 = time.perf_counter()
+        while steps < nt:
+            u = pyoracle.step(p, u, steps * dt, dt)
+            steps += 1
+            if time.perf_counter() - t0 > budget:
+                break
+        return steps, time.perf_counter() - t0
+
+    steps, el = run(threads, budget_s)
+    steps1, el1 = run(1, max(1.0, budget_s / 4))
     return {"value": p.nx * p.ny * steps / el, "unit": "cell-updates/s",
             "cores": threads, "kind": "port",
             "sample": f"{steps} of the {nt} fine steps of the plain sweep at "
-                      f"{p.nx}x{p.ny} ({el:.1f} s, OpenMP over lines)"}
+                      f"{p.nx}x{p.ny} ({el:.1f} s, OpenMP over lines)",
+            "single_thread_value": p.nx * p.ny * steps1 / el1,
+            "single_thread_sample": f"{steps1} fine steps on 1 thread ({el1:.1f} s)"}
 
 
 def rel_l2(a, b):
