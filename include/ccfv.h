@@ -242,12 +242,20 @@ int ccfv_parareal_solve(ccfv_ctx *ctx, double t0, double t1, int32_t nt_fine,
  *                       step and the first x sweep of the next run as one
  *                       launch on the same lines (one HBM round trip less);
  *                       0: one launch per sweep.  Both give identical
- *                       results. */
+ *                       results.
+ *   CCFV_OPT_PANEL      1 (default): the batched fine propagators keep their
+ *                       states between the first and the last x sweep of a
+ *                       run in a work buffer laid out as panels of 8 columns
+ *                       (contiguous x tiles, whole cache lines for y tiles),
+ *                       where the uniform-offset sweeps apply and the grid
+ *                       is a multiple of 8 both ways; 0: row-major.  Both
+ *                       give identical results. */
 #define CCFV_OPT_PIPELINE     1
 #define CCFV_OPT_PIPE_GROUP   2
 #define CCFV_OPT_GRAPHS       3
 #define CCFV_OPT_PLAIN_GRAPHS 4
 #define CCFV_OPT_FUSE         5
+#define CCFV_OPT_PANEL        6
 int ccfv_set_option(ccfv_ctx *ctx, int32_t key, int64_t value);
 int ccfv_get_option(ccfv_ctx *ctx, int32_t key, int64_t *value);
 

@@ -51,6 +51,7 @@ OPT_PIPE_GROUP = 2
 OPT_GRAPHS = 3
 OPT_PLAIN_GRAPHS = 4
 OPT_FUSE = 5
+OPT_PANEL = 6
 
 
 class Problem(ctypes.Structure):

@@ -49,6 +49,13 @@ struct ccfv_ctx {
     bool plain_graphs = false;
     // one launch for the last x sweep of a step and the first of the next
     bool fuse = true;
+    // batched fine propagators keep their intermediate states in the panel
+    // layout (SweepArgs::in_panel) in this work buffer of panel_fields
+    // fields; the run's first x sweep reads the row-major input and its last
+    // x sweep writes the row-major result
+    bool panel_on = true;
+    double *panel = nullptr;
+    int panel_fields = 0;
     int tiles_version = 0;
     std::map<std::string, hipGraphExec_t> gcache;
     std::vector<hipEvent_t> sync_ev;  // cross-stream events (no timing)

@@ -54,6 +54,12 @@ struct SweepArgs {
     const double *tab;
     const double *lineS, *lineK;  // uniform path: shift and 6 filter weights per line
     int vel_kind;
+    // panel layout of the input: the field as column panels of PANEL_W
+    // columns, cell (i, j) at (j / PANEL_W) * PANEL_W * nx + i * PANEL_W +
+    // j % PANEL_W.  The host picks the kernel instance by these flags (the
+    // uniform-offset batch sweeps, x tiles of PANEL_W lines); kernels do not
+    // read them
+    int in_panel;
     double v0, v1, v2;  // velocity parameters
     // time: field f of the launch sits at tn = t0 + (field0 + f)*tDT, the
     // sub-step starts at tl = tn + jdt and the velocity is sampled at
@@ -76,9 +82,14 @@ struct SweepArgs {
     // source term (CCFV_SRC_*): theta*h (src_w) times the remapped earlier
     // and the later cell averages
     int src_kind;
+    // panel layout of the output (out, or out2 when fused); see in_panel
+    int out_panel;
     double src[5];
     double src_w;
 };
+
+// columns per panel of the panel layout
+constexpr int PANEL_W = 8;
 
 struct Tile {
     int w;   // lines per block

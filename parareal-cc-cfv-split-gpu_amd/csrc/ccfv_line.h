@@ -868,4 +868,71 @@ __device__ __forceinline__ void store_tile_y(double *out, int ld, int n, int W,
     }
 }
 
+// ---------------------------------------------------------------------------
+// panel layout (SweepArgs::in_panel / out_panel): column panels of PANEL_W
+// columns, each stored row by row, so that the x tile of lines j0..j0+7 is
+// one contiguous block of 8 n doubles and a y tile of W rows reads 8 W
+// contiguous doubles (whole 128-byte lines for W >= 2) from every panel
+// ---------------------------------------------------------------------------
+// x tiles (W == PANEL_W, j0 a multiple of PANEL_W): the block of tile j0
+// starts at j0 * n and holds row i at PANEL_W * i, which is the row-major
+// addressing of load_tile_x / store_tile_x with row length PANEL_W and
+// base f + j0 * (n - 1)
+__device__ __forceinline__ const double *panel_x_base(const double *f, int n, int j0)
+{
+    return f + (size_t)j0 * (size_t)(n - 1);
+}
+__device__ __forceinline__ double *panel_x_base(double *f, int n, int j0)
+{
+    return f + (size_t)j0 * (size_t)(n - 1);
+}
+
+// y tiles of W rows i0.. (W a power of two, n a multiple of PANEL_W): ps =
+// PANEL_W * rows of the field.  double2 q of the tile is pair e = q mod 4W
+// of panel p = q / 4W: row r = e / 4, columns 2 (e mod 4) and +1, at
+// p ps + PANEL_W (i0 + r) + 2 (e mod 4); in LDS line r, pidx(8 p + c) =
+// 9 p + c.
+__device__ __forceinline__ void load_tile_y_panel(const double *f, size_t ps, int n, int i0,
+                                                  int W, double *lds, int LP)
+{
+    static_assert(PANEL_W == 8, "pidx chunks are panels");
+    const int sw = log2i(W) + 2, em = (1 << sw) - 1;
+    const int total = W * (n >> 1), step = blockDim.x;
+    const double *base = f + (size_t)PANEL_W * i0;
+    for (int b0 = threadIdx.x; b0 < total; b0 += TILE_UNROLL * step) {
+        double2 v[TILE_UNROLL];
+#pragma unroll
+        for (int u = 0; u < TILE_UNROLL; u++) {
+            const int q = b0 + u * step;
+            if (q < total)
+                v[u] = *reinterpret_cast<const double2 *>(base + (size_t)(q >> sw) * ps +
+                                                          2 * (q & em));
+        }
+#pragma unroll
+        for (int u = 0; u < TILE_UNROLL; u++) {
+            const int q = b0 + u * step;
+            if (q < total) {
+                const int e = q & em;
+                double *d = lds + (e >> 2) * LP + 9 * (q >> sw) + 2 * (e & 3);
+                d[0] = v[u].x;
+                d[1] = v[u].y;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void store_tile_y_panel(double *f, size_t ps, int n, int i0,
+                                                   int W, const double *lds, int LP)
+{
+    const int sw = log2i(W) + 2, em = (1 << sw) - 1;
+    const int total = W * (n >> 1);
+    double *base = f + (size_t)PANEL_W * i0;
+    for (int q = threadIdx.x; q < total; q += blockDim.x) {
+        const int e = q & em;
+        const double *d = lds + (e >> 2) * LP + 9 * (q >> sw) + 2 * (e & 3);
+        *reinterpret_cast<double2 *>(base + (size_t)(q >> sw) * ps + 2 * e) =
+            make_double2(d[0], d[1]);
+    }
+}
+
 }  // namespace ccfv

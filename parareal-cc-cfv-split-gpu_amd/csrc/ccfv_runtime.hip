@@ -337,6 +337,8 @@ hipError_t run_sweep(ccfv_ctx *c, Dir dir, bool post, Epi epi, const SweepArgs &
 {
     if (nf <= 0) return hipSuccess;
     if (a.lineS) return launch_sweep_fast(dir, epi, a, nf, t, c->cur);
+    // the general sweeps read and write row-major fields only
+    if (a.in_panel || a.out_panel) return hipErrorInvalidValue;
     return launch_sweep_generic(dir, c->P.bc, post, epi, a, nf, t, c->cur);
 }
 
@@ -369,6 +371,7 @@ hipError_t run_sweep_fused(ccfv_ctx *c, bool post, Epi epi, const SweepArgs &a, 
 {
     if (nf <= 0) return hipSuccess;
     if (a.lineS) return launch_sweep_fast(DIR_X, epi, a, nf, t, c->cur, true);
+    if (a.in_panel || a.out_panel) return hipErrorInvalidValue;
     return launch_sweep_generic(DIR_X, c->P.bc, post, epi, a, nf, t, c->cur, true);
 }
 
@@ -419,6 +422,39 @@ hipError_t timed_sweep_fused(ccfv_ctx *c, int kind, double bytes, bool post, Epi
 // of step i and the first x sweep of step i+1 are one launch over the same
 // lines: the same arithmetic on the same values, one HBM round trip and one
 // launch fewer per step.
+// the panel layout needs uniform-offset sweeps in both directions (the
+// generic sweeps read row-major fields only), x batch tiles of PANEL_W
+// lines, whole panels in both directions and a work buffer of nf fields
+bool panel_ready(const ccfv_ctx *c, const SolveTable &tx, const SolveTable &ty, int nf)
+{
+    return c->panel_on && c->panel && nf <= c->panel_fields && tx.line_dev && ty.line_dev &&
+           c->tile[DIR_X][1].w == PANEL_W && c->P.nx % PANEL_W == 0 &&
+           c->P.ny % PANEL_W == 0;
+}
+
+// the panel work buffer for batches of up to nf fields stepped by h; left
+// unallocated (batches then sweep row-major) where the layout cannot apply
+// or the memory is not there
+int ensure_panel(ccfv_ctx *c, int nf, double h)
+{
+    const SolveTable *tx, *ty;
+    int rc;
+    if ((rc = get_table(c, DIR_X, h, &tx)) || (rc = get_table(c, DIR_Y, h, &ty))) return rc;
+    if (!c->panel_on || nf < 2 || !tx->line_dev || !ty->line_dev ||
+        c->P.nx % PANEL_W || c->P.ny % PANEL_W || c->panel_fields >= nf)
+        return CCFV_OK;
+    (void)hipFree(c->panel);
+    c->panel = nullptr;
+    c->panel_fields = 0;
+    if (hipMalloc(&c->panel, sizeof(double) * c->ncell * (size_t)nf) != hipSuccess) {
+        (void)hipGetLastError();
+        c->panel = nullptr;
+        return CCFV_OK;
+    }
+    c->panel_fields = nf;
+    return CCFV_OK;
+}
+
 int run_steps(ccfv_ctx *c, int count, const std::function<StepDesc(int)> &desc,
               const std::function<int(int)> &pre, const std::function<int(int)> &post)
 {
@@ -430,10 +466,21 @@ int run_steps(ccfv_ctx *c, int count, const std::function<StepDesc(int)> &desc,
     const SolveTable *tx, *ty;
     if ((rc = get_table(c, DIR_X, D.h, &tx)) || (rc = get_table(c, DIR_Y, D.h, &ty)))
         return rc;
+    // a batch of uniform-offset sweeps keeps its states between the run's
+    // first and last x sweeps in the panel work buffer; every step of the run
+    // has the same step size and field count
+    bool panel = D.nf > 1 && panel_ready(c, *tx, *ty, D.nf) && D.epi == EPI_NONE;
+    for (int i = 1; panel && i < count; i++) {
+        const StepDesc S = desc(i);
+        panel = S.h == D.h && S.nf == D.nf && S.epi == EPI_NONE;
+    }
+    // where the x1 sweep of a step writes, and the y and x3 sweeps work
+    auto work = [&](const StepDesc &S) { return panel ? c->panel : S.out; };
     auto x1_args = [&](const StepDesc &S, const SolveTable &T) {
         SweepArgs a = make_args(c, DIR_X, 1, T, S.h, S.t0, S.tDT, S.field0, S.jdt);
         a.in = S.in;
-        a.out = S.out;
+        a.out = work(S);
+        a.out_panel = panel;
         a.fstride = fs;
         return a;
     };
@@ -447,14 +494,16 @@ int run_steps(ccfv_ctx *c, int count, const std::function<StepDesc(int)> &desc,
     for (int i = 0; i < count; i++) {
         const bool batch = D.nf > 1;
         SweepArgs b = make_args(c, DIR_Y, 2, *ty, D.h, D.t0, D.tDT, D.field0, D.jdt);
-        b.in = D.out;
-        b.out = D.out;
+        b.in = work(D);
+        b.out = work(D);
+        b.in_panel = b.out_panel = panel;
         b.fstride = fs;
         HIPCHK(timed_sweep(c, batch ? CCFV_KSTAT_Y_BATCH : CCFV_KSTAT_Y_SINGLE,
                            16.0 * (double)c->ncell * D.nf, DIR_Y, false, EPI_NONE, b, D.nf,
                            c->tile[DIR_Y][batch]));
         SweepArgs x3 = make_args(c, DIR_X, 3, *tx, D.h, D.t0, D.tDT, D.field0, D.jdt);
-        x3.in = D.out;
+        x3.in = work(D);
+        x3.in_panel = panel;
         x3.out = D.out;
         x3.fstride = fs;
         x3.g_cache = D.g_cache;
@@ -474,7 +523,9 @@ int run_steps(ccfv_ctx *c, int count, const std::function<StepDesc(int)> &desc,
                           N.tDT == D.tDT;
         if (pre && (rc = pre(i + 1))) return rc;
         if (fuse) {
-            x3.out2 = N.out;
+            // the next step's x1 result: its work field
+            x3.out2 = work(N);
+            x3.out_panel = panel;
             x3.field0_2 = N.field0;
             x3.t0_2 = N.t0;
             x3.jdt2 = N.jdt;
@@ -572,6 +623,9 @@ void free_para(ccfv_ctx *c)
     (void)hipFree(c->slot_sums);
     if (c->h_sums) (void)hipHostFree(c->h_sums);
     c->U = c->G = c->F = c->Wk = c->partial = c->slot_sums = c->h_sums = nullptr;
+    (void)hipFree(c->panel);
+    c->panel = nullptr;
+    c->panel_fields = 0;
     c->para_ns = 0;
 }
 
@@ -719,6 +773,7 @@ int ccfv_create(const ccfv_problem *p, ccfv_ctx **out)
     if (const char *s = getenv("CCFV_GRAPHS")) c->graphs = atoi(s) != 0;
     if (const char *s = getenv("CCFV_PLAIN_GRAPHS")) c->plain_graphs = atoi(s) != 0;
     if (const char *s = getenv("CCFV_FUSE")) c->fuse = atoi(s) != 0;
+    if (const char *s = getenv("CCFV_PANEL")) c->panel_on = atoi(s) != 0;
     (void)hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream);
     (void)hipMemsetAsync(c->state, 0, sizeof(double) * c->ncell, c->stream);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) {
@@ -1631,7 +1686,8 @@ int ccfv_parareal_solve(ccfv_ctx *c, double t0, double t1, int32_t nt_fine,
     // tables up front (host-side work stays out of the timed region)
     const SolveTable *tt;
     if ((rc = get_table(c, DIR_X, pl.DT, &tt)) || (rc = get_table(c, DIR_Y, pl.DT, &tt)) ||
-        (rc = get_table(c, DIR_X, pl.dt, &tt)) || (rc = get_table(c, DIR_Y, pl.dt, &tt)))
+        (rc = get_table(c, DIR_X, pl.dt, &tt)) || (rc = get_table(c, DIR_Y, pl.dt, &tt)) ||
+        (rc = ensure_panel(c, ns, pl.dt)))
         return rc;
     ccfv_parareal_report R;
     memset(&R, 0, sizeof(R));
@@ -1698,6 +1754,10 @@ int ccfv_set_option(ccfv_ctx *c, int32_t key, int64_t value)
         c->fuse = value != 0;
         clear_graphs(c);
         return CCFV_OK;
+    case CCFV_OPT_PANEL:
+        c->panel_on = value != 0;
+        clear_graphs(c);
+        return CCFV_OK;
     }
     return fail(CCFV_EINVAL, "unknown option");
 }
@@ -1711,6 +1771,7 @@ int ccfv_get_option(ccfv_ctx *c, int32_t key, int64_t *value)
     case CCFV_OPT_GRAPHS: *value = c->graphs ? 1 : 0; return CCFV_OK;
     case CCFV_OPT_PLAIN_GRAPHS: *value = c->plain_graphs ? 1 : 0; return CCFV_OK;
     case CCFV_OPT_FUSE: *value = c->fuse ? 1 : 0; return CCFV_OK;
+    case CCFV_OPT_PANEL: *value = c->panel_on ? 1 : 0; return CCFV_OK;
     }
     return fail(CCFV_EINVAL, "unknown option");
 }

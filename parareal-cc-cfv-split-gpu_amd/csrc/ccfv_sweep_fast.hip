@@ -477,12 +477,17 @@ __device__ __forceinline__ void fast_x_direct(const SweepArgs &a, int field, int
 // (one step size) and the same per-line shift, and go to out2.
 // direct: the launch sweeps one field alone (not a batch, not a ragged
 // launch over many problems), where the per-lane direct access pays; a
-// bandwidth-bound launch of many fields keeps the coalesced tile loads
-template <int CH, int EPI, bool FUSE>
+// bandwidth-bound launch of many fields keeps the coalesced tile loads.
+// LAYOUT: LAYOUT_IN / LAYOUT_OUT bits, input / output (out or out2) in the
+// panel layout (tiles of PANEL_W lines)
+constexpr int LAYOUT_IN = 1, LAYOUT_OUT = 2;
+template <int CH, int EPI, bool FUSE, int LAYOUT = 0>
 __device__ __forceinline__ void fast_x_body(const SweepArgs &a, int field, int tile,
                                            int W, int NW, double *lds, bool direct = false)
 {
-    if constexpr (CH >= 8 && EPI == EPI_NONE) {
+    static_assert(LAYOUT == 0 || EPI == EPI_NONE, "epilogues write row-major fields");
+    constexpr bool pin = (LAYOUT & LAYOUT_IN) != 0, pout = (LAYOUT & LAYOUT_OUT) != 0;
+    if constexpr (CH >= 8 && EPI == EPI_NONE && LAYOUT == 0) {
         if (direct && a.n == 64 * CH && W == NW) {
             fast_x_direct<CH, FUSE>(a, field, tile, W, lds);
             return;
@@ -503,15 +508,18 @@ __device__ __forceinline__ void fast_x_body(const SweepArgs &a, int field, int t
     const LineK first = line_k(a, j0 + wave);
     EpiInputs pre;
     if (EPI == EPI_CORRECT) epi_load(a, n, j0, W, threadIdx.x, pre);
-    load_tile_x(a.in + foff, a.ld, n, j0, W, lds, LP);
+    if constexpr (pin) load_tile_x(panel_x_base(a.in + foff, n, j0), PANEL_W, n, j0, W, lds, LP);
+    else load_tile_x(a.in + foff, a.ld, n, j0, W, lds, LP);
     __syncthreads();
     sweep_lines<CH, REG>(a, lds, LP, j0, W, NW, wave, lane, first, tb, mfl, mbl, pneg0);
     // the second sweep of a fused launch without an epilogue touches only the
     // wave's own lines: no barrier before it
     if (!FUSE || EPI != EPI_NONE) __syncthreads();
     if (!FUSE) {
-        if (EPI == EPI_NONE) store_tile_x(a.out + foff, a.ld, n, j0, W, lds, LP);
-        else epilogue_x<EPI>(a, n, j0, W, lds, LP, tile, &pre);
+        if constexpr (EPI != EPI_NONE) epilogue_x<EPI>(a, n, j0, W, lds, LP, tile, &pre);
+        else if constexpr (pout)
+            store_tile_x(panel_x_base(a.out + foff, n, j0), PANEL_W, n, j0, W, lds, LP);
+        else store_tile_x(a.out + foff, a.ld, n, j0, W, lds, LP);
         return;
     }
     if (EPI != EPI_NONE) {
@@ -520,7 +528,9 @@ __device__ __forceinline__ void fast_x_body(const SweepArgs &a, int field, int t
     }
     sweep_lines<CH, REG>(a, lds, LP, j0, W, NW, wave, lane, first, tb, mfl, mbl, pneg0);
     __syncthreads();
-    store_tile_x(a.out2 + foff, a.ld, n, j0, W, lds, LP);
+    if constexpr (pout)
+        store_tile_x(panel_x_base(a.out2 + foff, n, j0), PANEL_W, n, j0, W, lds, LP);
+    else store_tile_x(a.out2 + foff, a.ld, n, j0, W, lds, LP);
 }
 
 // y lines of 64*CH cells (CH >= 8), one wave per line (W == NW, the
@@ -583,11 +593,12 @@ __device__ __forceinline__ void fast_y_direct(const SweepArgs &a, int field, int
     }
 }
 
-template <int CH>
+template <int CH, int LAYOUT = 0>
 __device__ __forceinline__ void fast_y_body(const SweepArgs &a, int field, int tile,
                                            int W, int NW, double *lds, bool direct = false)
 {
-    if constexpr (CH >= 8) {
+    constexpr bool pin = (LAYOUT & LAYOUT_IN) != 0, pout = (LAYOUT & LAYOUT_OUT) != 0;
+    if constexpr (CH >= 8 && LAYOUT == 0) {
         if (direct && a.n == 64 * CH && W == NW) {
             fast_y_direct<CH>(a, field, tile, W, NW, lds);
             return;
@@ -604,26 +615,30 @@ __device__ __forceinline__ void fast_y_body(const SweepArgs &a, int field, int t
     const double mfl = a.tab[4 * n + lane], mbl = a.tab[4 * n + 64 + lane];
     const double pneg0 = a.tab[2 * n];
     const LineK first = line_k(a, i0 + wave);
-    load_tile_y(a.in + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
+    // panel stride: PANEL_W doubles per row of the field (a.nl rows)
+    const size_t ps = (size_t)PANEL_W * (size_t)a.nl;
+    if constexpr (pin) load_tile_y_panel(a.in + foff, ps, n, i0, W, lds, LP);
+    else load_tile_y(a.in + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
     __syncthreads();
     sweep_lines<CH, REG>(a, lds, LP, i0, W, NW, wave, lane, first, tb, mfl, mbl, pneg0);
     __syncthreads();
-    store_tile_y(a.out + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
+    if constexpr (pout) store_tile_y_panel(a.out + foff, ps, n, i0, W, lds, LP);
+    else store_tile_y(a.out + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
 }
 
-template <int CH, int EPI, bool FUSE>
+template <int CH, int EPI, bool FUSE, int LAYOUT = 0>
 __global__ __launch_bounds__(256) void fast_x_kernel(SweepArgs a, int W, int NW)
 {
     extern __shared__ double lds[];
-    fast_x_body<CH, EPI, FUSE>(a, blockIdx.y, xcd_tile(blockIdx.x, gridDim.x), W, NW,
-                               lds, gridDim.y == 1);
+    fast_x_body<CH, EPI, FUSE, LAYOUT>(a, blockIdx.y, xcd_tile(blockIdx.x, gridDim.x), W,
+                                       NW, lds, gridDim.y == 1);
 }
 
-template <int CH>
+template <int CH, int LAYOUT = 0>
 __global__ __launch_bounds__(256) void fast_y_kernel(SweepArgs a, int W, int NW)
 {
     extern __shared__ double lds[];
-    fast_y_body<CH>(a, blockIdx.y, blockIdx.x, W, NW, lds, gridDim.y == 1);
+    fast_y_body<CH, LAYOUT>(a, blockIdx.y, blockIdx.x, W, NW, lds, gridDim.y == 1);
 }
 
 // ragged: one launch over many problems of the same line class
@@ -682,6 +697,15 @@ template <int CH, bool FUSE>
 static hipError_t dispatch_x(Epi epi, dim3 grid, dim3 block, size_t lds, hipStream_t s,
                              const SweepArgs &a, Tile t)
 {
+    const int layout = (a.in_panel ? LAYOUT_IN : 0) | (a.out_panel ? LAYOUT_OUT : 0);
+    if (layout && (epi != EPI_NONE || t.w != PANEL_W)) return hipErrorInvalidValue;
+    if (layout == LAYOUT_IN)
+        return launch(fast_x_kernel<CH, EPI_NONE, FUSE, LAYOUT_IN>, grid, block, lds, s, a, t);
+    if (layout == LAYOUT_OUT)
+        return launch(fast_x_kernel<CH, EPI_NONE, FUSE, LAYOUT_OUT>, grid, block, lds, s, a, t);
+    if (layout)
+        return launch(fast_x_kernel<CH, EPI_NONE, FUSE, LAYOUT_IN | LAYOUT_OUT>, grid, block,
+                      lds, s, a, t);
     if (epi == EPI_NONE)
         return launch(fast_x_kernel<CH, EPI_NONE, FUSE>, grid, block, lds, s, a, t);
     if (epi == EPI_INIT)
@@ -695,7 +719,13 @@ static hipError_t dispatch(Dir dir, Epi epi, bool fuse, const SweepArgs &a, int 
 {
     dim3 grid(a.nl / t.w, nf), block(64 * t.nw);
     size_t lds = sweep_lds_bytes_fast(a.n, t);
-    if (dir == DIR_Y) return launch(fast_y_kernel<CH>, grid, block, lds, s, a, t);
+    if (dir == DIR_Y) {
+        if (a.in_panel != a.out_panel || (a.in_panel && a.n % PANEL_W))
+            return hipErrorInvalidValue;
+        if (a.in_panel)
+            return launch(fast_y_kernel<CH, LAYOUT_IN | LAYOUT_OUT>, grid, block, lds, s, a, t);
+        return launch(fast_y_kernel<CH>, grid, block, lds, s, a, t);
+    }
     return fuse ? dispatch_x<CH, true>(epi, grid, block, lds, s, a, t)
                 : dispatch_x<CH, false>(epi, grid, block, lds, s, a, t);
 }

@@ -106,3 +106,47 @@ def test_fused_x_launches_change_nothing(gpu, make):
             out[fuse] = res
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("make", [
+    lambda: (T.rotation_problem(64), T.rotation_ic()),
+    lambda: T.constant_gaussians(64, 128, -0.7, 0.4, 3e-3,
+                                 [(0.3, 0.6, 0.05, 1.0), (0.7, 0.2, 0.08, 0.6)]),
+    lambda: (T.advection_sinsin_problem(128), T.advection_sinsin_ic()),
+])
+@pytest.mark.parametrize("pipe,fuse,graphs", [(0, 1, 0), (1, 1, 0), (1, 0, 0), (1, 1, 1),
+                                              (2, 1, 0)])
+def test_panel_layout_changes_nothing(gpu, make, pipe, fuse, graphs):
+    """Fine-propagator batches with their states in 8-column panels (default)
+    against row-major: bitwise equal final state and increments, for every
+    schedule, with and without fused x launches and captured phases."""
+    p, ic = make()
+    out = []
+    for panel in (0, 1):
+        with gpu.Context(p) as c:
+            c.set_option(T.OPT_PANEL, panel)
+            assert c.get_option(T.OPT_PANEL) == panel
+            c.set_option(T.OPT_PIPELINE, pipe)
+            c.set_option(T.OPT_FUSE, fuse)
+            c.set_option(T.OPT_GRAPHS, graphs)
+            c.set_option(T.OPT_PIPE_GROUP, 3)
+            c.fill_exact(ic, 0.0)
+            rep = c.parareal_solve(0.0, 0.5, 64, 8, 4, 0.0)
+            out.append((c.get_state(), list(rep.increments)))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+
+
+def test_panel_layout_full_size(gpu):
+    """The 512^2 rotation with 64 slices of 16 fine steps (groups of 32
+    slices), two iterations: panel layout against row-major, bitwise."""
+    p, ic = T.rotation_problem(512), T.rotation_ic()
+    out = []
+    for panel in (0, 1):
+        with gpu.Context(p) as c:
+            c.set_option(T.OPT_PANEL, panel)
+            c.fill_exact(ic, 0.0)
+            rep = c.parareal_solve(0.0, 1.0, 1024, 64, 2, 0.0)
+            out.append((c.get_state(), list(rep.increments)))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
