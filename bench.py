@@ -44,8 +44,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # that make copy_kernel of a known size read its true bytes), over one
 # Parareal iteration of this workload on one stream
 # (profiles/rotation512_fine_sweep_pmc.md)
-TRAFFIC_RATIO = {"x_batch": 1.002, "y_batch": 1.002, "y_single": 1.062,
-                 "x_epilog": 1.027}
+TRAFFIC_RATIO = {"x_batch": 1.002, "y_batch": 1.002, "y_single": 1.064,
+                 "x_epilog": 1.029}
 
 
 def parse():
