@@ -588,6 +588,18 @@ __device__ __forceinline__ int xcd_tile(int bx, int gx)
     return (bx & 7) * (gx >> 3) + (bx >> 3);
 }
 
+// The same for the blocks of one item of a ragged x launch: block `local` of
+// an item whose nblk blocks start at launch block `first` sweeps the item's
+// tile number (local mod 8) * nblk/8 + local/8 (tile number = field * tiles
+// per field + tile), so each XCD takes a contiguous run of the item's
+// column tiles.  Applies when first and nblk are multiples of 8 (then block
+// local sits on XCD local mod 8); a bijection of [0, nblk) either way.
+__device__ __forceinline__ int ragged_xcd_tile(int local, int first, int nblk)
+{
+    if ((first | nblk) & 7) return local;
+    return (local & 7) * (nblk >> 3) + (local >> 3);
+}
+
 // Loads of a tile are issued TILE_UNROLL at a time per work-item before any
 // of them is written to LDS, so each wave keeps several requests in flight.
 constexpr int TILE_UNROLL = 8;

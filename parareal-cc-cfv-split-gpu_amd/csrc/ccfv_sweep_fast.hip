@@ -650,7 +650,7 @@ __global__ __launch_bounds__(256) void ragged_fast_x(const RaggedItem *items,
     extern __shared__ double lds[];
     const int b = blockIdx.x;
     const int i = __builtin_amdgcn_readfirstlane(ragged_item(prefix, nitems, b));
-    const int local = b - prefix[i];
+    const int local = ragged_xcd_tile(b - prefix[i], prefix[i], prefix[i + 1] - prefix[i]);
     const int nt = items[i].ntiles;
     const int field = local / nt;
     const SweepArgs a = items[i].a;

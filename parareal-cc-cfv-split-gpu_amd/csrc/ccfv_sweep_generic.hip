@@ -241,7 +241,7 @@ __global__ __launch_bounds__(512) void ragged_x_kernel(const RaggedItem *items,
     extern __shared__ double lds[];
     const int b = blockIdx.x;
     const int i = ragged_item(prefix, nitems, b);
-    const int local = b - prefix[i];
+    const int local = ragged_xcd_tile(b - prefix[i], prefix[i], prefix[i + 1] - prefix[i]);
     const int nt = items[i].ntiles;
     const int field = local / nt;
     // the arguments stay in global memory (scalar-cached); a private copy of
