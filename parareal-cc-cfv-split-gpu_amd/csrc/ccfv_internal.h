@@ -112,7 +112,20 @@ struct RaggedItem {
 hipError_t launch_sweep_generic(Dir dir, int bc, bool post, Epi epi,
                                 const SweepArgs &a, int nfields, Tile tile,
                                 hipStream_t s, bool fuse = false);
-size_t sweep_lds_bytes_generic(int n, Tile tile, bool src = false);
+// Whether a general sweep keeps the RHS of a lane's cells in registers
+// (rhs_general_reg) or stages it in an auxiliary LDS line per wave
+// (rhs_general): registers for periodic, source-free sweeps, which then fit
+// in VGPRs without spilling at every chunk size (the closed box's ghosts and
+// the source's terms already fill the registers); never for the
+// third-sub-step variant, which needs the whole remapped line.  Kernels and
+// the host's LDS sizes follow the same rule.
+__host__ __device__ constexpr bool rhs_in_regs(int ch, int bc, bool src, bool post)
+{
+    return (void)ch, !post && bc == CCFV_BC_PERIODIC && !src;
+}
+// aux: the kernel stages line data in an auxiliary LDS line per wave (the
+// third-sub-step variant, or an RHS kept out of registers: rhs_in_regs)
+size_t sweep_lds_bytes_generic(int n, Tile tile, bool src, bool aux = true);
 // uniform-displacement sweeps (ccfv_sweep_fast.hip); periodic only
 hipError_t launch_sweep_fast(Dir dir, Epi epi, const SweepArgs &a, int nfields,
                              Tile tile, hipStream_t s, bool fuse = false);

@@ -424,6 +424,46 @@ __device__ void rhs_general(const SweepArgs &a, const Times &tt, const double *c
     if (bad) *a.flag = 1;
 }
 
+// The same RHS without the third-sub-step variant, straight into the lane's
+// registers: b[k] for its CH cells.  The cells are visited in a loop (the
+// face evaluation is too large to unroll CH times) that writes with a
+// wave-uniform runtime index, in parts of at most 16 cells with an array of
+// their own each: arrays of up to 16 doubles indexed that way stay in VGPRs
+// (32 go to scratch).  Same arithmetic, in the same order, as rhs_general.
+template <int DIR, int BC, bool SRC, int CH>
+__device__ void rhs_general_reg(const SweepArgs &a, const Times &tt, const double *cs,
+                                const double g[4], double (&b)[CH], int base,
+                                const LineGeo &lg, double inv12d, const double *fs,
+                                const double *fg, int line, double t1)
+{
+    constexpr int P = CH < 16 ? CH : 16, NP = CH / P;
+    bool bad = false;
+    Face fl = eval_face<DIR, BC, false>(a, tt, cs, g, base, lg, inv12d);
+    const bool src = SRC && fs;
+    Face gl;
+    if (src) gl = eval_face<DIR, BC, true>(a, tt, fs, fg, base, lg, inv12d);
+#pragma unroll
+    for (int c = 0; c < NP; c++) {
+        double part[P];
+#pragma unroll 1
+        for (int k = 0; k < P; k++) {
+            const int e = base + c * P + k;
+            Face fr = eval_face<DIR, BC, false>(a, tt, cs, g, e + 1, lg, inv12d);
+            double cbar = remap_cell<BC>(a, cs, fl, fr, bad);
+            if (src) {
+                Face gr = eval_face<DIR, BC, true>(a, tt, fs, fg, e + 1, lg, inv12d);
+                cbar = cbar + src_rhs<DIR, BC>(a, tt, fs, fg, lg, line, e, gl, gr, t1, bad);
+                gl = gr;
+            }
+            part[k] = cbar + a.coef * (fr.D - fl.D);
+            fl = fr;
+        }
+#pragma unroll
+        for (int k = 0; k < P; k++) b[c * P + k] = part[k];
+    }
+    if (bad) *a.flag = 1;
+}
+
 // ---------------------------------------------------------------------------
 // block reduction of two doubles into partial[2*slot ...]
 // ---------------------------------------------------------------------------

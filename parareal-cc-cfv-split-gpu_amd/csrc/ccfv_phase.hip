@@ -78,8 +78,12 @@ void Phase::add_sweep_stage(std::vector<StageItem> &items)
             it.push_back(s->item);
             prefix.push_back(prefix.back() + s->item.nfields * s->item.ntiles);
             size_t b = k.fast ? sweep_lds_bytes_fast(s->item.a.n, s->tile)
-                              : sweep_lds_bytes_generic(s->item.a.n, s->tile,
-                                                        s->item.a.src_kind != CCFV_SRC_NONE);
+                              : sweep_lds_bytes_generic(
+                                    s->item.a.n, s->tile, s->item.a.src_kind != CCFV_SRC_NONE,
+                                    (s->dir == DIR_X && k.bc == CCFV_BC_REFERENCE && k.post) ||
+                                        !rhs_in_regs(k.ch, k.bc,
+                                                     s->item.a.src_kind != CCFV_SRC_NONE,
+                                                     false));
             lds = std::max(lds, b);
         }
         Launch L;

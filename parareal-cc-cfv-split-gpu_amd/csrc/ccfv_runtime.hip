@@ -60,7 +60,9 @@ bool tile_ok(Tile t, int n, int nl, Dir dir, bool post, bool src)
     if (t.w <= 0 || t.nw <= 0 || t.nw > 4 || t.w % t.nw || nl % t.w) return false;
     if (t.w & (t.w - 1)) return false;  // power of two: tiles split by shifts
     if (dir == DIR_X && (t.w & 1)) return false;
-    return std::max(sweep_lds_bytes_generic(n, t, src), sweep_lds_bytes_fast(n, t)) <=
+    // the general sweeps' largest layout (with the auxiliary line), whatever
+    // the boundary mode
+    return std::max(sweep_lds_bytes_generic(n, t, src, true), sweep_lds_bytes_fast(n, t)) <=
            160 * 1024;
 }
 

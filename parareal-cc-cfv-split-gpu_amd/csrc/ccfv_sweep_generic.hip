@@ -36,11 +36,12 @@ CCFV_GEN_DECL(32)
 
 #ifndef CCFV_GEN_CH
 
-size_t sweep_lds_bytes_generic(int n, Tile tile, bool src)
+size_t sweep_lds_bytes_generic(int n, Tile tile, bool src, bool aux)
 {
-    // W tile lines + one auxiliary line per wave (+ one source line per wave)
+    // W tile lines (+ one auxiliary line per wave for the third-sub-step
+    // variant) (+ one source line per wave)
     return sizeof(double) * (size_t)line_pitch(n) *
-           (size_t)(tile.w + tile.nw * (src ? 2 : 1));
+           (size_t)(tile.w + tile.nw * ((aux ? 1 : 0) + (src ? 1 : 0)));
 }
 
 hipError_t launch_sweep_generic(Dir dir, int bc, bool post, Epi epi,
@@ -83,7 +84,10 @@ hipError_t launch_ragged_generic(Dir dir, int bc, bool post, Epi epi, int ch, bo
 #else  // CCFV_GEN_CH: the kernels of one chunk size
 
 // cs: the line in LDS (pidx layout), overwritten with the result.
-// aux: a line-sized LDS buffer owned by this wave.
+// aux: a line-sized LDS buffer owned by this wave, where the RHS is staged
+// unless rhs_in_regs (and, POST, the remapped line whose derivative the
+// third sub-step takes); not allocated when every sweep of the kernel keeps
+// the RHS in registers.
 // fs: a second line-sized LDS buffer of this wave, for the source's earlier
 // cell averages (st: the source's times).  SRC: the problem has a source
 // (a.src_kind != 0); the source-free instance carries none of its code.
@@ -110,10 +114,14 @@ __device__ void process_line(const SweepArgs &a, const Times tt, double *cs, dou
         if (BC == BC_REF) ghosts(fs, n, fg);
     }
 
+    constexpr bool regs = rhs_in_regs(CH, BC, SRC, POST);
     double b[CH];
-    if (active)
+    if (regs && active)
+        rhs_general_reg<DIR, BC, SRC, CH>(a, tt, cs, g, b, base, lg, inv12d,
+                                          src ? fs : nullptr, fg, line, st.t1);
+    if (!regs && active)
         rhs_general<DIR, BC, SRC>(a, tt, cs, g, aux, base, CH, lg, inv12d, POST,
-                             src ? fs : nullptr, fg, line, st.t1);
+                                  src ? fs : nullptr, fg, line, st.t1);
     if (POST) {
         // the reference's third sub-step takes the derivative of the remapped
         // line (T2S4_Parareal.cu:929-932); the source joins afterwards
@@ -144,7 +152,7 @@ __device__ void process_line(const SweepArgs &a, const Times tt, double *cs, dou
             if (bad) *a.flag = 1;
         }
         __syncthreads();
-    } else if (active) {
+    } else if (!regs && active) {
 #pragma unroll
         for (int k = 0; k < CH; k++) b[k] = aux[pidx(base + k)];
     }
@@ -171,8 +179,11 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, int field, int 
     load_tile_x(a.in + foff, a.ld, n, j0, W, lds, LP);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // per wave: an auxiliary line (unless every sweep here keeps the RHS in
+    // registers), then a source line (SRC only)
+    constexpr bool aux_line = POST || !rhs_in_regs(CH, BC, SRC, false);
     double *aux = lds + (W + wave) * LP;
-    double *fs = lds + (W + NW + wave) * LP;  // allocated with a source only
+    double *fs = lds + (W + (aux_line ? NW : 0) + wave) * LP;
     const Times tt = sub_times(a, field);
     const SrcTimes st = src_times(a, field, false);
     for (int l = wave; l < W; l += NW)
@@ -206,8 +217,11 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs &a, int field, int 
     load_tile_y(a.in + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // per wave: an auxiliary line (unless the RHS stays in registers), then
+    // a source line (SRC only)
+    constexpr bool aux_line = !rhs_in_regs(CH, BC, SRC, false);
     double *aux = lds + (W + wave) * LP;
-    double *fs = lds + (W + NW + wave) * LP;  // allocated with a source only
+    double *fs = lds + (W + (aux_line ? NW : 0) + wave) * LP;
     const Times tt = sub_times(a, field);
     const SrcTimes st = src_times(a, field, false);
     for (int l = wave; l < W; l += NW)
@@ -306,7 +320,8 @@ static hipError_t dispatch_src(Dir dir, bool post, Epi epi, bool fuse, const Swe
                                int nf, Tile t, hipStream_t s)
 {
     dim3 grid(a.nl / t.w, nf), block(64 * t.nw);
-    size_t lds = sweep_lds_bytes_generic(a.n, t, SRC);
+    size_t lds = sweep_lds_bytes_generic(
+        a.n, t, SRC, (dir == DIR_X && BC == BC_REF && post) || !rhs_in_regs(CH, BC, SRC, false));
     if (dir == DIR_Y) return launch(sweep_y_kernel<BC, CH, SRC>, grid, block, lds, s, a, t);
     if (BC == BC_REF && post)
         return fuse ? dispatch_x<BC, CH, true, true, SRC>(epi, grid, block, lds, s, a, t)
