@@ -83,7 +83,10 @@ int hip_fail(hipError_t e, const char *what);
         if (e_ != hipSuccess) return ccfv::rt::hip_fail(e_, #expr); \
     } while (0)
 
-bool tile_ok(Tile t, int n, int nl, Dir dir, bool post, bool src = false);
+// bc: the boundary mode the tile is for; the closed box (default) is the
+// largest general-sweep layout, valid for either mode
+bool tile_ok(Tile t, int n, int nl, Dir dir, bool post, bool src = false,
+             int bc = CCFV_BC_REFERENCE);
 
 // Host range for rocprofv3 --marker-trace (a solve, a Parareal iteration, a
 // batch or serving step); without a tool attached the calls return at once.

@@ -53,16 +53,16 @@ Tile env_tile(const char *name, Tile dflt)
     return t;
 }
 
-bool tile_ok(Tile t, int n, int nl, Dir dir, bool post, bool src)
+bool tile_ok(Tile t, int n, int nl, Dir dir, bool post, bool src, int bc)
 {
-    (void)post;
     // NW <= 4: the sweep kernels are compiled for at most 256 work-items
     if (t.w <= 0 || t.nw <= 0 || t.nw > 4 || t.w % t.nw || nl % t.w) return false;
     if (t.w & (t.w - 1)) return false;  // power of two: tiles split by shifts
     if (dir == DIR_X && (t.w & 1)) return false;
-    // the general sweeps' largest layout (with the auxiliary line), whatever
-    // the boundary mode
-    return std::max(sweep_lds_bytes_generic(n, t, src, true), sweep_lds_bytes_fast(n, t)) <=
+    // the general sweeps' layout: with the auxiliary line unless the RHS stays
+    // in registers (the closed box, the default, always has it)
+    const bool aux = (dir == DIR_X && post) || !rhs_in_regs(chunk_for(n), bc, src, false);
+    return std::max(sweep_lds_bytes_generic(n, t, src, aux), sweep_lds_bytes_fast(n, t)) <=
            160 * 1024;
 }
 
@@ -744,7 +744,7 @@ int ccfv_create(const ccfv_problem *p, ccfv_ctx **out)
     for (int d = 0; d < 2; d++)
         for (int b = 0; b < 2; b++)
             if (!tile_ok(c->tile[d][b], d == DIR_X ? p->nx : p->ny,
-                         d == DIR_X ? p->ny : p->nx, (Dir)d, post && d == DIR_X, src)) {
+                         d == DIR_X ? p->ny : p->nx, (Dir)d, post && d == DIR_X, src, p->bc)) {
                 delete c;
                 return fail(CCFV_EINVAL, "no valid tile for this grid");
             }
@@ -1005,7 +1005,8 @@ int ccfv_set_tiles(ccfv_ctx *c, const int32_t t[8])
         v[i] = Tile{t[2 * i], t[2 * i + 1]};
         Dir d = i < 2 ? DIR_X : DIR_Y;
         int n = d == DIR_X ? c->P.nx : c->P.ny, nl = d == DIR_X ? c->P.ny : c->P.nx;
-        if (!tile_ok(v[i], n, nl, d, post && d == DIR_X, c->P.src_kind != CCFV_SRC_NONE))
+        if (!tile_ok(v[i], n, nl, d, post && d == DIR_X, c->P.src_kind != CCFV_SRC_NONE,
+                     c->P.bc))
             return fail(CCFV_EINVAL, "invalid tile shape for this grid");
     }
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1079,7 +1080,7 @@ int ccfv_autotune(ccfv_ctx *c, int32_t batch_fields, double h, int32_t out[8])
                 for (int nw : nws) {
                     Tile t{w, nw};
                     if (!tile_ok(t, n, nl, dir, post && dir == DIR_X,
-                                 c->P.src_kind != CCFV_SRC_NONE))
+                                 c->P.src_kind != CCFV_SRC_NONE, c->P.bc))
                         continue;
                     float tbest = 1e30f;
                     for (int trial = 0; trial < 2; trial++) {
