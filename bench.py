@@ -68,7 +68,9 @@ def parse():
                          "capture of the 2048-launch plain solve)")
     ap.add_argument("--probe", action="store_true",
                     help="short run for counter collection: one 32 MiB "
-                         "calibration copy and one Parareal iteration")
+                         "calibration copy and one Parareal iteration "
+                         "(--workload swirl: 64 plain steps and one Parareal "
+                         "iteration, one stream, no graphs)")
     ap.add_argument("--workload", default="rotation",
                     choices=("rotation", "ragged", "serve", "swirl", "copy"),
                     help="rotation: the Parareal benchmark (default); ragged: "
@@ -295,6 +297,24 @@ def swirl(a):
     p, ic = T.swirl_problem(n, T=5.0, nu=1e-4), T.cosine_bell_ic()
     ctx = ccfv.Context(p)
     ctx.fill_exact(ic, 0.0)
+    if a.probe:
+        # short run for kernel traces: 64 plain steps and one Parareal
+        # iteration on one stream, default launch shapes, no graphs
+        ctx.set_option(T.OPT_GRAPHS, 0)
+        ctx.set_option(T.OPT_PIPELINE, 0)
+        t = time.perf_counter()
+        ctx.plain_solve(0.0, 5.0 * 64 / nt, 64)
+        ctx.sync()
+        plain_ms = (time.perf_counter() - t) * 1e3
+        ctx.fill_exact(ic, 0.0)
+        t = time.perf_counter()
+        r = ctx.parareal_solve(0.0, 5.0, nt, ns, 1, 0.0)
+        ctx.sync()
+        print(json.dumps({"probe": "ok", "workload": f"swirl-{n}", "plain_64_ms": plain_ms,
+                          "parareal_k1_ms": (time.perf_counter() - t) * 1e3,
+                          "iterations": r.iterations}))
+        ctx.close()
+        return
     u0 = ctx.get_state()
     t_tune = time.perf_counter()
     tiles = ctx.autotune(ns, 5.0 / nt)
