@@ -17,8 +17,12 @@ namespace ccfv {
 
 // Thomas factors of this lane's fixed cell positions.  For the symmetric
 // matrix -p_e == mneg_e for 0 < e < n-1, so one array serves both passes
-// (pneg_0 is a scalar, pneg_{n-1} = 0).  Small chunks keep them in registers
-// for every line a wave sweeps; large chunks read the global table.
+// (pneg_0 is a scalar, pneg_{n-1} = 0).  Chunks of up to 16 cells keep them
+// in registers for every line a wave sweeps (16-cell chunks neither in the
+// coarse-chain epilogue launches, whose registers would then cost a wave per
+// SIMD, nor in the panel-layout batches that run beside the chain: 1024^2
+// pipelined Parareal 97 instead of 82 ms); larger chunks read the global
+// table.
 template <int CH, bool REG>
 struct LaneTab;
 
@@ -436,7 +440,7 @@ __device__ __forceinline__ void fast_x_direct(const SweepArgs &a, int field, int
                                              int W, double *lds)
 {
     static_assert(CH >= 8, "full-wave lines of whole chunks");
-    constexpr bool REG = CH <= 8;
+    constexpr bool REG = CH <= 16;
     constexpr int n = 64 * CH;
     const int LP = line_pitch(n + fast_ghost_cells(CH));
     const size_t foff = (size_t)field * (size_t)a.fstride;
@@ -493,7 +497,7 @@ __device__ __forceinline__ void fast_x_body(const SweepArgs &a, int field, int t
             return;
         }
     }
-    constexpr bool REG = CH <= 8;
+    constexpr bool REG = CH <= 8 || (CH <= 16 && EPI == EPI_NONE && LAYOUT == 0);
     const int n = a.n, LP = line_pitch(n + fast_ghost_cells(CH));
     const size_t foff = (size_t)field * (size_t)a.fstride;
     const int j0 = tile * W;
@@ -547,7 +551,7 @@ __device__ __forceinline__ void fast_y_direct(const SweepArgs &a, int field, int
                                              int W, int NW, double *lds)
 {
     static_assert(CH >= 8, "full-wave lines of whole chunks");
-    constexpr bool REG = CH <= 8;
+    constexpr bool REG = CH <= 16;
     constexpr int n = 64 * CH;
     const int LP = line_pitch(n + fast_ghost_cells(CH));
     const size_t foff = (size_t)field * (size_t)a.fstride;
@@ -604,7 +608,7 @@ __device__ __forceinline__ void fast_y_body(const SweepArgs &a, int field, int t
             return;
         }
     }
-    constexpr bool REG = CH <= 8;
+    constexpr bool REG = CH <= 8 || (CH <= 16 && LAYOUT == 0);
     const int n = a.n, LP = line_pitch(n + fast_ghost_cells(CH));
     const size_t foff = (size_t)field * (size_t)a.fstride;
     const int i0 = tile * W;
