@@ -119,9 +119,10 @@ hipError_t launch_sweep_generic(Dir dir, int bc, bool post, Epi epi,
 // the source's terms already fill the registers); never for the
 // third-sub-step variant, which needs the whole remapped line.  Kernels and
 // the host's LDS sizes follow the same rule.
-__host__ __device__ constexpr bool rhs_in_regs(int ch, int bc, bool src, bool post)
+__host__ __device__ constexpr bool rhs_in_regs([[maybe_unused]] int ch, int bc, bool src,
+                                               bool post)
 {
-    return (void)ch, !post && bc == CCFV_BC_PERIODIC && !src;
+    return !post && bc == CCFV_BC_PERIODIC && !src;
 }
 // aux: the kernel stages line data in an auxiliary LDS line per wave (the
 // third-sub-step variant, or an RHS kept out of registers: rhs_in_regs)
