@@ -66,6 +66,10 @@ def parse():
                          "(for rocprofv3 --kernel-trace runs: the profiler's "
                          "process ends in a segmentation fault inside the "
                          "capture of the 2048-launch plain solve)")
+    ap.add_argument("--ring", type=int, default=8,
+                    help="--workload copy: fields of 32 MiB in the ring (8: 256 MiB, "
+                         "more than the last-level cache holds; 4: 128 MiB, the "
+                         "size of one batch of 64 fine propagators at 512^2)")
     ap.add_argument("--probe", action="store_true",
                     help="short run for counter collection: one 32 MiB "
                          "calibration copy and one Parareal iteration "
@@ -277,21 +281,23 @@ def serve(a):
 def copy_rate(a):
     """Copies for the practical HBM ceiling: the library's copy kernel
     (ccfv_get_state_device) from one 2048^2 field (32 MiB) to the next, round
-    a ring of 8 fields (256 MiB), 16 B moved per double.  Each call
+    a ring of --ring fields (default 8: 256 MiB), 16 B moved per double.  Each call
     synchronizes, so the rate comes from the kernel durations under
     rocprofv3 --kernel-trace, not from a host clock."""
     import ccfv
     from ccfv import types as T
     big = T.rotation_problem(2048)
-    ring = [ccfv.Context(big) for _ in range(8)]
+    nr = max(2, a.ring)
+    ring = [ccfv.Context(big) for _ in range(nr)]
     for c in ring:
         c.fill_exact(T.rotation_ic(), 0.0)
         c.sync()
     ptrs = [c.state_ptr() for c in ring]
-    copies = 8 * max(1, a.steps)
-    for i in range(8 * max(1, a.warmup) + copies):
-        ring[i % 8].get_state_device(ptrs[(i + 1) % 8])
-    print(json.dumps({"workload": "copy", "copies": copies + 8 * max(1, a.warmup),
+    copies = nr * max(1, a.steps)
+    for i in range(nr * max(1, a.warmup) + copies):
+        ring[i % nr].get_state_device(ptrs[(i + 1) % nr])
+    print(json.dumps({"workload": "copy", "ring_fields": nr,
+                      "copies": copies + nr * max(1, a.warmup),
                       "bytes_per_copy": 16 * 2048 * 2048,
                       "kernel": "ccfv::copy_kernel"}))
 
