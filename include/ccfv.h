@@ -150,6 +150,11 @@ typedef struct ccfv_parareal_report {
     double  fine_steps_run;      /* single-field fine steps executed           */
     double  coarse_steps_run;    /* single-field coarse steps executed         */
     double  wall_ms;             /* hipEvent time of the whole solve           */
+    int32_t serial_finish;       /* 1: kmax was reached above tol and the result
+                                  * is the serial fine solve from the initial
+                                  * state (CCFV_OPT_SERIAL_FINISH); converged
+                                  * is then 1                                  */
+    int32_t reserved;
 } ccfv_parareal_report;
 
 typedef struct ccfv_ctx ccfv_ctx;
@@ -256,6 +261,16 @@ int ccfv_parareal_solve(ccfv_ctx *ctx, double t0, double t1, int32_t nt_fine,
 #define CCFV_OPT_PLAIN_GRAPHS 4
 #define CCFV_OPT_FUSE         5
 #define CCFV_OPT_PANEL        6
+/*   CCFV_OPT_SERIAL_FINISH 1: a solve that reaches kmax with tol > 0 and an
+ *                       increment still above tol finishes with the serial
+ *                       fine solve of the same nt_fine steps from its initial
+ *                       state (the steps of ccfv_plain_solve), so the result
+ *                       always meets the accuracy contract: it is then the
+ *                       serial fine solution itself; the report says
+ *                       serial_finish = 1, converged = 1 and the call returns
+ *                       CCFV_OK.  0 (default): the reference's behaviour, the
+ *                       last iterate and CCFV_ENOTCONV. */
+#define CCFV_OPT_SERIAL_FINISH 7
 int ccfv_set_option(ccfv_ctx *ctx, int32_t key, int64_t value);
 int ccfv_get_option(ccfv_ctx *ctx, int32_t key, int64_t *value);
 
@@ -329,6 +344,10 @@ int  ccfv_batch_plain(ccfv_batch *b, const double *t0, const double *t1,
 int  ccfv_batch_parareal(ccfv_batch *b, const double *t0, const double *t1,
                          const int32_t *nt_fine, const int32_t *n_slices,
                          int32_t kmax, double tol, ccfv_parareal_report *reports);
+/* ccfv_set_option on every member context (CCFV_OPT_SERIAL_FINISH: problems
+ * of ccfv_batch_parareal that reach kmax above tol finish with their serial
+ * fine solve, in the same lock-step rounds as the others). */
+int  ccfv_batch_set_option(ccfv_batch *b, int32_t key, int64_t value);
 
 /* ---- serving engine ------------------------------------------------------
  * (no counterpart in the reference, which solves one problem per process.)
@@ -352,6 +371,13 @@ typedef struct ccfv_server_config {
     int32_t  adaptive_above; /* > 0: a Parareal request submitted while this
                               * many requests are queued or resident runs as
                               * the plain serial solve (0: as requested)     */
+    int32_t  serial_finish;  /* 1: a Parareal request that reaches kmax above
+                              * its tol continues as its serial fine solve
+                              * from its initial state, still resident, so
+                              * every result meets its tol (as
+                              * CCFV_OPT_SERIAL_FINISH); 0: it is returned
+                              * with CCFV_ENOTCONV                             */
+    int32_t  reserved;
 } ccfv_server_config;
 
 typedef struct ccfv_request {

@@ -91,6 +91,8 @@ _SIGS = {
                                            ctypes.POINTER(ctypes.c_int32),
                                            ctypes.c_int32, ctypes.c_double,
                                            ctypes.POINTER(ParaReport)]),
+    "ccfv_batch_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
+                                             ctypes.c_int64]),
     "ccfv_request_bytes": (ctypes.c_int, [ctypes.POINTER(types.Request),
                                           ctypes.POINTER(ctypes.c_uint64)]),
     "ccfv_server_create": (ctypes.c_int, [ctypes.POINTER(types.ServerConfig),
@@ -252,6 +254,13 @@ class Context:
         _check(rc, "ccfv_parareal_solve")
         return rep
 
+    def parareal_correction(self, g_new: int, g_old: int, f: int, u: int, n: int):
+        """u = g_new + (f - g_old) over n doubles at device addresses, queued
+        on the context's stream (ccfv_parareal_correction)."""
+        _check(load().ccfv_parareal_correction(
+            self._h, ctypes.c_void_p(g_new), ctypes.c_void_p(g_old), ctypes.c_void_p(f),
+            ctypes.c_void_p(u), ctypes.c_size_t(n)), "ccfv_parareal_correction")
+
     def sync(self):
         _check(load().ccfv_sync(self._h), "ccfv_sync")
 
@@ -356,6 +365,10 @@ class Batch:
         _check(load().ccfv_batch_size(self._h, ctypes.byref(n)), "ccfv_batch_size")
         return n.value
 
+    def set_option(self, key: int, value: int):
+        """ccfv_set_option on every member (e.g. OPT_SERIAL_FINISH)."""
+        _check(load().ccfv_batch_set_option(self._h, key, value), "ccfv_batch_set_option")
+
     def plain(self, t0, t1, nt):
         n = len(self)
         a0 = _per_problem(t0, n, np.float64)
@@ -395,9 +408,10 @@ class Server:
     solve requests under a device-memory budget."""
 
     def __init__(self, mem_budget: int = 0, max_active: int = 0,
-                 max_slices: int = 0, max_bypass: int = 0, adaptive_above: int = 0):
+                 max_slices: int = 0, max_bypass: int = 0, adaptive_above: int = 0,
+                 serial_finish: int = 0):
         cfg = types.ServerConfig(mem_budget, max_active, max_slices, max_bypass,
-                                 adaptive_above)
+                                 adaptive_above, serial_finish)
         h = ctypes.c_void_p()
         _check(load().ccfv_server_create(ctypes.byref(cfg), ctypes.byref(h)),
                "ccfv_server_create")

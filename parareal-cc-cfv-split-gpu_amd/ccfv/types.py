@@ -52,6 +52,7 @@ OPT_GRAPHS = 3
 OPT_PLAIN_GRAPHS = 4
 OPT_FUSE = 5
 OPT_PANEL = 6
+OPT_SERIAL_FINISH = 7
 
 
 class Problem(ctypes.Structure):
@@ -118,6 +119,7 @@ class ParaReport(ctypes.Structure):
         ("fine_steps_run", ctypes.c_double),
         ("coarse_steps_run", ctypes.c_double),
         ("wall_ms", ctypes.c_double),
+        ("serial_finish", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
     @property
@@ -129,7 +131,8 @@ class ServerConfig(ctypes.Structure):
     """ccfv_server_config."""
     _fields_ = [("mem_budget", ctypes.c_uint64), ("max_active", ctypes.c_int32),
                 ("max_slices", ctypes.c_int32), ("max_bypass", ctypes.c_int32),
-                ("adaptive_above", ctypes.c_int32)]
+                ("adaptive_above", ctypes.c_int32), ("serial_finish", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class Request(ctypes.Structure):

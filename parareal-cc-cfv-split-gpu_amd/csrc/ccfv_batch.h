@@ -92,7 +92,13 @@ void shared_tiles(ccfv_ctx *c);
 struct ParaJob {
     ccfv_ctx *c = nullptr;
     int ns = 0, J = 0, nb = 0, kmax = 0;
-    double t0 = 0, DT = 0, dt = 0, tol = 0;
+    double t0 = 0, t1 = 0, DT = 0, dt = 0, tol = 0;
+    int nt_fine = 0;
+    // reaching kmax above tol switches the job to its serial fine solve
+    // from U[0] (the initial state, never written by the iterations):
+    // plain = true from then on, fine steps counted after para_fine
+    bool serial_finish = false;
+    double para_fine = 0;
     const SolveTable *fx = nullptr, *fy = nullptr, *cx = nullptr, *cy = nullptr;
     int k = 0;            // next iteration
     bool started = false; // k = 0 coarse sweep issued
@@ -122,7 +128,10 @@ int job_setup_plain(ParaJob &j, ccfv_ctx *c, double t0, double t1, int nt);
 int append_iteration(Phase &ph, std::vector<ParaJob *> &jobs, double *sums_dev,
                      size_t *used);
 // reads an iteration's slot sums (host copy of sums_dev) into the job; for
-// a plain job counts the steps of the phase
+// a plain job counts the steps of the phase.  A Parareal job that stops at
+// kmax above tol with serial_finish set becomes a plain job of nt_fine steps
+// of (t1 - t0) / nt_fine on U[0] (ccfv_plain_solve's steps) and keeps
+// running.
 void finish_iteration(ParaJob &j, const double *sums_host);
 
 }  // namespace rt

@@ -199,8 +199,7 @@ int ccfv_batch_parareal(ccfv_batch *b, const double *t0, const double *t1,
     }
     {
         std::vector<RaggedCopy> cp;
-        for (ParaJob &j : jobs)
-            cp.push_back(RaggedCopy{j.c->U + (size_t)j.ns * j.c->ncell, j.c->state, j.c->ncell});
+        for (ParaJob &j : jobs) cp.push_back(RaggedCopy{j.result(), j.c->state, j.c->ncell});
         ph.add_copies(cp);
         if ((rc = b->run.run(ph))) return rc;
     }
@@ -224,6 +223,16 @@ int ccfv_batch_parareal(ccfv_batch *b, const double *t0, const double *t1,
             if (!jobs[i].R.converged)
                 return fail(CCFV_ENOTCONV, "problem " + std::to_string(i) +
                                                " did not reach the tolerance within kmax");
+    return CCFV_OK;
+}
+
+int ccfv_batch_set_option(ccfv_batch *b, int32_t key, int64_t value)
+{
+    if (!b) return fail(CCFV_EINVAL, "NULL batch");
+    for (ccfv_ctx *c : b->ctx) {
+        int rc = ccfv_set_option(c, key, value);
+        if (rc) return rc;
+    }
     return CCFV_OK;
 }
 

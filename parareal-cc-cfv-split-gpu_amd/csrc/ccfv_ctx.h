@@ -54,6 +54,9 @@ struct ccfv_ctx {
     // fields; the run's first x sweep reads the row-major input and its last
     // x sweep writes the row-major result
     bool panel_on = true;
+    // a Parareal solve that reaches kmax above tol finishes with the serial
+    // fine solve from its initial state (CCFV_OPT_SERIAL_FINISH)
+    bool serial_finish = false;
     double *panel = nullptr;
     int panel_fields = 0;
     int tiles_version = 0;

@@ -342,6 +342,9 @@ int job_setup(ParaJob &j, ccfv_ctx *c, double t0, double t1, int nt_fine, int ns
     j.ns = ns;
     j.J = nt_fine / ns;
     j.t0 = t0;
+    j.t1 = t1;
+    j.nt_fine = nt_fine;
+    j.serial_finish = c->serial_finish;
     j.DT = (t1 - t0) / ns;
     j.dt = j.DT / (double)j.J;
     j.nb = c->chain_blocks;
@@ -490,7 +493,7 @@ void finish_iteration(ParaJob &p, const double *sums_host)
 {
     if (p.plain) {
         p.done = std::min(p.nt, p.done + PLAIN_CHUNK);
-        p.R.fine_steps_run = p.done;
+        p.R.fine_steps_run = p.para_fine + p.done;
         if (p.done >= p.nt) {
             p.R.converged = 1;  // the serial fine solution itself
             p.running = false;
@@ -525,6 +528,16 @@ void finish_iteration(ParaJob &p, const double *sums_host)
         p.running = false;
     }
     if (p.k >= p.kmax) p.running = false;
+    if (!p.running && !p.R.converged && p.tol > 0 && p.serial_finish) {
+        // the serial fine solve from the initial state in U[0]
+        p.plain = true;
+        p.nt = p.nt_fine;
+        p.done = 0;
+        p.dt = (p.t1 - p.t0) / p.nt;  // as ccfv_plain_solve
+        p.para_fine = p.R.fine_steps_run;
+        p.R.serial_finish = 1;
+        p.running = true;
+    }
 }
 
 }  // namespace rt

@@ -1686,10 +1686,15 @@ int ccfv_parareal_solve(ccfv_ctx *c, double t0, double t1, int32_t nt_fine,
     // iteration k fixes slot k+1; past n_slices iterations nothing changes
     pl.kmax = std::min((int)kmax, (int)ns);
     pl.tol = tol;
-    // tables up front (host-side work stays out of the timed region)
+    // tables up front (host-side work stays out of the timed region); the
+    // serial finish steps by (t1 - t0) / nt_fine as ccfv_plain_solve
+    const bool finish = c->serial_finish && tol > 0;
+    const double dt_plain = (t1 - t0) / nt_fine;
     const SolveTable *tt;
     if ((rc = get_table(c, DIR_X, pl.DT, &tt)) || (rc = get_table(c, DIR_Y, pl.DT, &tt)) ||
         (rc = get_table(c, DIR_X, pl.dt, &tt)) || (rc = get_table(c, DIR_Y, pl.dt, &tt)) ||
+        (finish && ((rc = get_table(c, DIR_X, dt_plain, &tt)) ||
+                    (rc = get_table(c, DIR_Y, dt_plain, &tt)))) ||
         (rc = ensure_panel(c, ns, pl.dt)))
         return rc;
     ccfv_parareal_report R;
@@ -1717,7 +1722,25 @@ int ccfv_parareal_solve(ccfv_ctx *c, double t0, double t1, int32_t nt_fine,
         if (rep) *rep = R;
         return rc;
     }
-    HIPCHK(launch_copy(c->U + (size_t)ns * N, c->state, N, c->stream));
+    if (finish && !converged) {
+        // kmax reached above tol: the serial fine solve from the initial
+        // state, still in U[0] (the steps of ccfv_plain_solve)
+        HIPCHK(launch_copy(c->U, c->state, N, c->stream));
+        rc = run_steps(c, nt_fine, [&](int n) {
+            return StepDesc{c->state, c->state, 1, t0 + n * dt_plain, 0.0, 0, 0.0, dt_plain};
+        });
+        if (rc) {
+            (void)hipStreamSynchronize(c->stream);
+            if (rep) *rep = R;
+            return rc;
+        }
+        R.fine_steps_run += nt_fine;
+        R.serial_finish = 1;
+        R.converged = 1;
+        converged = true;
+    } else {
+        HIPCHK(launch_copy(c->U + (size_t)ns * N, c->state, N, c->stream));
+    }
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     rc = check_flag(c);
     float ms = 0.f;
@@ -1761,6 +1784,10 @@ int ccfv_set_option(ccfv_ctx *c, int32_t key, int64_t value)
         c->panel_on = value != 0;
         clear_graphs(c);
         return CCFV_OK;
+    case CCFV_OPT_SERIAL_FINISH:
+        if (value != 0 && value != 1) return fail(CCFV_EINVAL, "serial finish must be 0 or 1");
+        c->serial_finish = value != 0;
+        return CCFV_OK;
     }
     return fail(CCFV_EINVAL, "unknown option");
 }
@@ -1775,6 +1802,7 @@ int ccfv_get_option(ccfv_ctx *c, int32_t key, int64_t *value)
     case CCFV_OPT_PLAIN_GRAPHS: *value = c->plain_graphs ? 1 : 0; return CCFV_OK;
     case CCFV_OPT_FUSE: *value = c->fuse ? 1 : 0; return CCFV_OK;
     case CCFV_OPT_PANEL: *value = c->panel_on ? 1 : 0; return CCFV_OK;
+    case CCFV_OPT_SERIAL_FINISH: *value = c->serial_finish ? 1 : 0; return CCFV_OK;
     }
     return fail(CCFV_EINVAL, "unknown option");
 }

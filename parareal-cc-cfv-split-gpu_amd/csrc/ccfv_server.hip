@@ -129,6 +129,7 @@ int place(ccfv_server *s, Request &q)
     c.partial = take(sizeof(double) * 2 * (size_t)c.chain_blocks * (ns + 1));
     c.para_ns = ns;
     c.state = nullptr;
+    c.serial_finish = s->cfg.serial_finish != 0;
     if ((size_t)(p - (s->arena + q.off)) > q.bytes)
         return fail(CCFV_EDEVICE, "request layout exceeds its arena block");
     if ((rc = job_setup(q.job, &c, q.r.t0, q.r.t1, q.r.nt_fine, ns, q.r.kmax, q.r.tol)))
@@ -197,6 +198,8 @@ int ccfv_server_create(const ccfv_server_config *cfg, ccfv_server **out)
     if (c.max_active < 1 || c.max_active > 4096) return fail(CCFV_EINVAL, "max_active must be in [1, 4096]");
     if (c.max_slices < 1 || c.max_slices > 1024) return fail(CCFV_EINVAL, "max_slices must be in [1, 1024]");
     if (c.max_bypass < 0) return fail(CCFV_EINVAL, "max_bypass must be >= 0");
+    if (c.serial_finish != 0 && c.serial_finish != 1)
+        return fail(CCFV_EINVAL, "serial_finish must be 0 or 1");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(CCFV_EDEVICE, "no HIP device available");

@@ -38,7 +38,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts():
     assert ctypes.sizeof(T.Problem) == 96 + 8 + 5 * 8
     assert ctypes.sizeof(T.IC) == 8 + 16 * 8
-    assert ctypes.sizeof(T.ParaReport) == 16 + 64 * 8 + 24
+    assert ctypes.sizeof(T.ParaReport) == 16 + 64 * 8 + 24 + 8
     assert ctypes.sizeof(T.KernelStat) == 24
 
 
