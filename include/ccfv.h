@@ -271,8 +271,20 @@ int ccfv_parareal_solve(ccfv_ctx *ctx, double t0, double t1, int32_t nt_fine,
  *                       CCFV_OK.  0 (default): the reference's behaviour, the
  *                       last iterate and CCFV_ENOTCONV. */
 #define CCFV_OPT_SERIAL_FINISH 7
+/*   CCFV_OPT_KEEP_ITERATES 1: ccfv_parareal_solve keeps the end state
+ *                       U^k(t1) of every iteration k in device memory (kmax
+ *                       fields, allocated by the solve) for
+ *                       ccfv_parareal_iterate; 0 (default): not kept. */
+#define CCFV_OPT_KEEP_ITERATES 8
 int ccfv_set_option(ccfv_ctx *ctx, int32_t key, int64_t value);
 int ccfv_get_option(ccfv_ctx *ctx, int32_t key, int64_t *value);
+
+/* End state U^k(t1) of iteration k (1 <= k <= iterations of the last
+ * ccfv_parareal_solve on this context, run with CCFV_OPT_KEEP_ITERATES) into
+ * a host array of nx*ny doubles: the iterate whose distance to the serial
+ * fine solution the reference prints after each iteration
+ * (T2S4_Parareal.cu:1333-1346), from one solve. */
+int ccfv_parareal_iterate(ccfv_ctx *ctx, int32_t k, double *host);
 
 /* U = G_new + (F - G_old) over n doubles, device pointers, on the context's
  * stream (parareal_correction_kernel, :771-782). */

@@ -67,6 +67,7 @@ _SIGS = {
                                            ctypes.POINTER(ParaReport)]),
     "ccfv_parareal_correction": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4
                                  + [ctypes.c_size_t]),
+    "ccfv_parareal_iterate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp]),
     "ccfv_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "ccfv_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64]),
     "ccfv_get_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
@@ -253,6 +254,13 @@ class Context:
             return rep
         _check(rc, "ccfv_parareal_solve")
         return rep
+
+    def parareal_iterate(self, k: int) -> np.ndarray:
+        """U^k(t1) of iteration k of the last parareal_solve (needs
+        OPT_KEEP_ITERATES on)."""
+        out = np.empty(self.problem.nx * self.problem.ny, dtype=np.float64)
+        _check(load().ccfv_parareal_iterate(self._h, k, _ptr(out)), "ccfv_parareal_iterate")
+        return out
 
     def parareal_correction(self, g_new: int, g_old: int, f: int, u: int, n: int):
         """u = g_new + (f - g_old) over n doubles at device addresses, queued

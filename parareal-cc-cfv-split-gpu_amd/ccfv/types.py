@@ -53,6 +53,7 @@ OPT_PLAIN_GRAPHS = 4
 OPT_FUSE = 5
 OPT_PANEL = 6
 OPT_SERIAL_FINISH = 7
+OPT_KEEP_ITERATES = 8
 
 
 class Problem(ctypes.Structure):

@@ -57,6 +57,11 @@ struct ccfv_ctx {
     // a Parareal solve that reaches kmax above tol finishes with the serial
     // fine solve from its initial state (CCFV_OPT_SERIAL_FINISH)
     bool serial_finish = false;
+    // every iteration's end state U^k(t1) kept in iter_buf (iter_cap fields;
+    // iter_valid of them from the last solve) (CCFV_OPT_KEEP_ITERATES)
+    bool keep_iterates = false;
+    double *iter_buf = nullptr;
+    int iter_cap = 0, iter_valid = 0;
     double *panel = nullptr;
     int panel_fields = 0;
     int tiles_version = 0;

@@ -793,6 +793,7 @@ void ccfv_destroy(ccfv_ctx *c)
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     if (c->stream3) (void)hipStreamSynchronize(c->stream3);
     free_para(c);
+    (void)hipFree(c->iter_buf);
     drain_records(c);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->sync_ev) (void)hipEventDestroy(e);
@@ -971,6 +972,18 @@ int ccfv_plain_solve(ccfv_ctx *c, double t0, double t1, int32_t nt)
         if (rc) return rc;
     }
     return check_flag(c);
+}
+
+int ccfv_parareal_iterate(ccfv_ctx *c, int32_t k, double *host)
+{
+    if (!c || !host) return fail(CCFV_EINVAL, "NULL argument");
+    if (k < 1 || k > c->iter_valid)
+        return fail(CCFV_EINVAL, "no kept iterate k: run ccfv_parareal_solve with "
+                                 "CCFV_OPT_KEEP_ITERATES and 1 <= k <= its iterations");
+    HIPCHK(hipMemcpyAsync(host, c->iter_buf + (size_t)(k - 1) * c->ncell,
+                          sizeof(double) * c->ncell, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CCFV_OK;
 }
 
 int ccfv_parareal_correction(ccfv_ctx *c, const double *g_new,
@@ -1178,6 +1191,15 @@ struct ParaPlan {
     double tol;
 };
 
+// U[ns] after iteration k -> kept iterate k+1 (CCFV_OPT_KEEP_ITERATES), on
+// the stream that ran the iteration's coarse chain
+hipError_t keep_iterate(ccfv_ctx *c, int ns, int k, hipStream_t s)
+{
+    if (!c->keep_iterates || !c->iter_buf || k >= c->iter_cap) return hipSuccess;
+    const size_t N = c->ncell;
+    return launch_copy(c->U + (size_t)ns * N, c->iter_buf + (size_t)k * N, N, s);
+}
+
 // the steps of the Parareal schedule
 StepDesc init_step(ccfv_ctx *c, const ParaPlan &pl, int n)
 {
@@ -1232,6 +1254,7 @@ int parareal_sequential(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R
         rc = run_steps(c, ns - k - 1, [&](int i) { return chain_step(c, pl, k + 1 + i); });
         if (rc) return rc;
         R.coarse_steps_run += ns - k - 1;
+        HIPCHK(keep_iterate(c, ns, k, c->stream));
         double *sums = c->slot_sums + (size_t)k * 2 * (ns + 1);
         HIPCHK(launch_reduce_segments(c->partial, nb, k + 1, ns - k, sums, c->stream));
         HIPCHK(hipMemcpyAsync(c->h_sums + (size_t)k * 2 * (ns + 1), sums,
@@ -1368,6 +1391,7 @@ int parareal_pipelined(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R,
                 return CCFV_OK;
             });
         if (r2) return r2;
+        HIPCHK(keep_iterate(c, ns, k, sC));
         double *sums = c->slot_sums + (size_t)k * 2 * (ns + 1);
         HIPCHK(launch_reduce_segments(c->partial, nb, k + 1, ns - k, sums, sC));
         HIPCHK(hipMemcpyAsync(c->h_sums + (size_t)k * 2 * (ns + 1), sums,
@@ -1571,6 +1595,7 @@ int parareal_overlapped(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R
             });
         if (r2) return r2;
         HIPCHK(launch_copy(c->U + (size_t)ns * N, res[par], N, sC));
+        HIPCHK(keep_iterate(c, ns, k, sC));
         if (k + 1 < ns) HIPCHK(hipEventRecord(ev(npar, 0, G - 1), sC));
         double *sums = c->slot_sums + (size_t)k * 2 * (ns + 1);
         HIPCHK(launch_reduce_segments(part[par], nb, k + 1, ns - k, sums, sC));
@@ -1697,6 +1722,19 @@ int ccfv_parareal_solve(ccfv_ctx *c, double t0, double t1, int32_t nt_fine,
                     (rc = get_table(c, DIR_Y, dt_plain, &tt)))) ||
         (rc = ensure_panel(c, ns, pl.dt)))
         return rc;
+    c->iter_valid = 0;
+    if (c->keep_iterates && c->iter_cap < pl.kmax) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        (void)hipFree(c->iter_buf);
+        c->iter_buf = nullptr;
+        c->iter_cap = 0;
+        clear_graphs(c);  // captured phases hold the old buffer's address
+        if (hipMalloc(&c->iter_buf, sizeof(double) * N * (size_t)pl.kmax) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(CCFV_ENOMEM, "buffer of kept Parareal iterates");
+        }
+        c->iter_cap = pl.kmax;
+    }
     ccfv_parareal_report R;
     memset(&R, 0, sizeof(R));
     R.n_slices = ns;
@@ -1714,6 +1752,7 @@ int ccfv_parareal_solve(ccfv_ctx *c, double t0, double t1, int32_t nt_fine,
     c->cur = c->stream;
     if (rc == CCFV_OK && kdone == ns) converged = true;  // every slice exact
     R.iterations = kdone;
+    if (c->keep_iterates) c->iter_valid = std::min(kdone, c->iter_cap);
     R.converged = converged ? 1 : 0;
     if (rc) {
         (void)hipStreamSynchronize(c->stream);
@@ -1788,6 +1827,12 @@ int ccfv_set_option(ccfv_ctx *c, int32_t key, int64_t value)
         if (value != 0 && value != 1) return fail(CCFV_EINVAL, "serial finish must be 0 or 1");
         c->serial_finish = value != 0;
         return CCFV_OK;
+    case CCFV_OPT_KEEP_ITERATES:
+        if (value != 0 && value != 1) return fail(CCFV_EINVAL, "keep iterates must be 0 or 1");
+        c->keep_iterates = value != 0;
+        c->iter_valid = 0;
+        clear_graphs(c);
+        return CCFV_OK;
     }
     return fail(CCFV_EINVAL, "unknown option");
 }
@@ -1803,6 +1848,7 @@ int ccfv_get_option(ccfv_ctx *c, int32_t key, int64_t *value)
     case CCFV_OPT_FUSE: *value = c->fuse ? 1 : 0; return CCFV_OK;
     case CCFV_OPT_PANEL: *value = c->panel_on ? 1 : 0; return CCFV_OK;
     case CCFV_OPT_SERIAL_FINISH: *value = c->serial_finish ? 1 : 0; return CCFV_OK;
+    case CCFV_OPT_KEEP_ITERATES: *value = c->keep_iterates ? 1 : 0; return CCFV_OK;
     }
     return fail(CCFV_EINVAL, "unknown option");
 }

@@ -35,10 +35,13 @@ def read_field(path, n):
     return np.array(vals)
 
 
-def test_cli_run_matches_the_library(gpu, tmp_path):
+@pytest.mark.parametrize("t0", [0.0, 0.1])
+def test_cli_run_matches_the_library(gpu, tmp_path, t0):
+    """t0 = T_min: the initial data are the exact solution at t = 0 either
+    way (T2S4_Parareal.cu:1147); the steps start at T_min."""
     n, nt, span, K, t1 = 32, 80, 20, 0.01, 0.5
     (tmp_path / "config.txt").write_text(
-        f"T_min:0\nT_max:{t1}\nN:{n}\nNt:{nt}\nT_span:{span}\nK:{K}\n")
+        f"T_min:{t0}\nT_max:{t1}\nN:{n}\nNt:{nt}\nT_span:{span}\nK:{K}\n")
     r = subprocess.run([CLI, "config.txt"], cwd=tmp_path, capture_output=True, text=True,
                        timeout=300)
     assert r.returncode == 0, r.stderr
@@ -47,7 +50,7 @@ def test_cli_run_matches_the_library(gpu, tmp_path):
     iters = [l for l in out if l.startswith("Parareal iter")]
     assert len(iters) == 8
     assert out[-8:-1] == ["K:0.01", "dx:0.0625000000000000", "dy:0.0625000000000000",
-                          "dt:0.0062500000000000", "Nx:32", "Ny:32", "Nt:80"]
+                          f"dt:{(t1 - t0) / nt:.16f}", "Nx:32", "Ny:32", "Nt:80"]
     assert out[-1].startswith("Total time (including Parareal): ")
     files = sorted(p.name for p in tmp_path.glob("data_T2S4_*.txt"))
     assert files == [f"data_T2S4_{s:06d}.txt" for s in range(0, nt + 1, span)]
@@ -59,19 +62,30 @@ def test_cli_run_matches_the_library(gpu, tmp_path):
                                    rtol=6e-5, atol=1e-300)
         snaps = {}
         for s in range(span, nt + 1, span):
-            c.advance(0.0, t1 / nt, s - span, s)
+            c.advance(t0, (t1 - t0) / nt, s - span, s)
             snaps[s] = c.get_state()
         for s, u in snaps.items():
             np.testing.assert_allclose(read_field(tmp_path / f"data_T2S4_{s:06d}.txt", n),
                                        u, rtol=6e-5, atol=1e-300)
         uref = snaps[nt]
+        # the k-th line is the k-th iterate of one solve, which is also the
+        # end state of a solve stopped after k iterations
+        c.set_option(T.OPT_KEEP_ITERATES, 1)
+        c.set_state(u0)
+        c.parareal_solve(t0, t1, nt, 40, 8, 0.0)
+        kept = {}
         for k, l in enumerate(iters, start=1):
             m = re.match(r"^Parareal iter +(\d+): err\(T\) = (\S+)$", l)
             assert m and int(m.group(1)) == k
-            c.set_state(u0)
-            c.parareal_solve(0.0, t1, nt, 40, k, 0.0)
-            err = float(np.abs(c.get_state() - uref).max())
+            kept[k] = c.parareal_iterate(k)
+            err = float(np.abs(kept[k] - uref).max())
             assert m.group(2) == f"{err:.3e}", (k, l, err)
+        assert np.array_equal(c.get_state(), kept[8])
+        c.set_option(T.OPT_KEEP_ITERATES, 0)
+        for k in (1, 3):
+            c.set_state(u0)
+            c.parareal_solve(t0, t1, nt, 40, k, 0.0)
+            assert np.array_equal(c.get_state(), kept[k]), k
 
 
 def test_advance_is_the_plain_loop(gpu):

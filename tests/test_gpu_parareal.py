@@ -150,3 +150,26 @@ def test_panel_layout_full_size(gpu):
             out.append((c.get_state(), list(rep.increments)))
     assert np.array_equal(out[0][0], out[1][0])
     assert out[0][1] == out[1][1]
+
+
+@pytest.mark.parametrize("mode,graphs", [(0, 0), (1, 0), (1, 1), (2, 0)])
+def test_kept_iterates_are_the_stopped_solves(gpu, mode, graphs):
+    """OPT_KEEP_ITERATES: iterate k of one solve equals, bitwise, the end
+    state of a solve stopped after k iterations, in every schedule."""
+    p, ic = T.rotation_problem(64), T.rotation_ic()
+    kmax = 5
+    with gpu.Context(p) as c:
+        c.set_option(T.OPT_PIPELINE, mode)
+        c.set_option(T.OPT_GRAPHS, graphs)
+        c.set_option(T.OPT_PIPE_GROUP, 3)
+        c.set_option(T.OPT_KEEP_ITERATES, 1)
+        c.fill_exact(ic, 0.0)
+        rep = c.parareal_solve(0.0, 0.5, 64, 8, kmax, 0.0)
+        assert rep.iterations == kmax
+        kept = [c.parareal_iterate(k) for k in range(1, kmax + 1)]
+        assert np.array_equal(kept[-1], c.get_state())
+        with pytest.raises(gpu.CcfvError):
+            c.parareal_iterate(kmax + 1)
+    for k in range(1, kmax + 1):
+        u, _ = solve(gpu, p, ic, 0, 3, k, 0.0)
+        assert np.array_equal(kept[k - 1], u), k
