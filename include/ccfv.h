@@ -333,6 +333,36 @@ int ccfv_profile_enable(ccfv_ctx *ctx, int32_t enable);
 /* Fill out[CCFV_KSTAT_COUNT]; synchronises the stream. */
 int ccfv_profile_read(ccfv_ctx *ctx, ccfv_kernel_stat *out);
 
+/* Mean launch time of one sweep class in isolation (instrumentation; no
+ * counterpart in the reference): `launches` back-to-back sweeps of direction
+ * dir (0 = x, 1 = y) over nfields fields of scratch memory holding copies of
+ * the state (the state is not changed), step size h, between two hipEvents
+ * on the context's stream.  fused = 1 (x only): the launch a run of steps
+ * issues, the last x sub-step of one step and the first x sub-step of the
+ * next.  layout 0: row-major fields; 1: the panel layout (8-column panels,
+ * uniform-displacement sweeps only; x shapes 8 lines wide).  tile_w, tile_nw:
+ * the launch shape (0, 0: the context's shape for that many fields).
+ * graph = 1: the launches are captured once as a hipGraph and the replay is
+ * timed (no host submission cost per launch).  Each launch reads and writes
+ * every cell of every field once: 16 B per cell and field.  *us_per_launch
+ * includes the gaps between back-to-back launches. */
+int ccfv_time_sweep(ccfv_ctx *ctx, int32_t dir, int32_t fused, int32_t nfields, double h,
+                    int32_t launches, int32_t layout, int32_t tile_w, int32_t tile_nw,
+                    int32_t graph, double *us_per_launch);
+
+/* Launch-rate floor of the one-field sweeps (instrumentation; no counterpart
+ * in the reference).  `launches` back-to-back launches on the context's
+ * stream of, for direction dir (0 = x, 1 = y) and the context's one-field
+ * launch shape (one wave per line):
+ *   kind 0: an empty kernel with that grid;
+ *   kind 1: a copy of the state field into a scratch field with the sweep's
+ *           memory pattern (y: each lane moves its contiguous chunk of a row;
+ *           x: each lane its strided chunk of a column) and no arithmetic.
+ * graph = 1 captures the launches as one hipGraph and times its replay.
+ * *us_per_launch: hipEvent time / launches.  The state is not changed. */
+int ccfv_dispatch_floor(ccfv_ctx *ctx, int32_t dir, int32_t kind, int32_t launches,
+                        int32_t graph, double *us_per_launch);
+
 /* ---- batches of independent problems -----------------------------------
  * (workloads of many independent problems; no counterpart in the reference, which solves
  * one hard-coded problem).  Problems may differ in grid (multiples of 8),

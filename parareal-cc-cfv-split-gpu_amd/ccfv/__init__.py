@@ -69,6 +69,13 @@ _SIGS = {
                                  + [ctypes.c_size_t]),
     "ccfv_parareal_iterate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp]),
     "ccfv_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "ccfv_time_sweep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
+                                       ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
+    "ccfv_dispatch_floor": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.POINTER(ctypes.c_double)]),
     "ccfv_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64]),
     "ccfv_get_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
                                        ctypes.POINTER(ctypes.c_int64)]),
@@ -271,6 +278,24 @@ class Context:
 
     def sync(self):
         _check(load().ccfv_sync(self._h), "ccfv_sync")
+
+    def time_sweep(self, direction: int, fused: bool, nfields: int, h: float,
+                   launches: int, layout: int = 0, tile=(0, 0), graph: bool = False) -> float:
+        """µs per launch of back-to-back sweeps of one class (ccfv_time_sweep)."""
+        us = ctypes.c_double()
+        _check(load().ccfv_time_sweep(self._h, direction, int(fused), nfields, h, launches,
+                                      layout, tile[0], tile[1], int(graph), ctypes.byref(us)),
+               "ccfv_time_sweep")
+        return us.value
+
+    def dispatch_floor(self, direction: int, kind: int, launches: int,
+                       graph: bool = False) -> float:
+        """µs per launch of back-to-back empty (kind 0) or copy (kind 1)
+        launches with the one-field sweep's grid (ccfv_dispatch_floor)."""
+        us = ctypes.c_double()
+        _check(load().ccfv_dispatch_floor(self._h, direction, kind, launches, int(graph),
+                                          ctypes.byref(us)), "ccfv_dispatch_floor")
+        return us.value
 
     def set_option(self, key: int, value: int):
         _check(load().ccfv_set_option(self._h, key, value), "ccfv_set_option")

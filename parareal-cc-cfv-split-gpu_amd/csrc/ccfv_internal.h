@@ -181,5 +181,10 @@ hipError_t launch_fill_exact(const ccfv_problem &p, const ccfv_ic &ic, double t,
 hipError_t launch_copy(const double *src, double *dst, size_t n, hipStream_t s);
 hipError_t launch_nonfinite_check(const double *src, size_t n, int *flag,
                                   hipStream_t s);
+// launch-rate floor (instrumentation): kind 0 an empty kernel, kind 1 a copy
+// in -> out with the one-field sweep's memory pattern of direction dir; one
+// wave per line of n cells, nl lines, row length ld, t.nw waves per block
+hipError_t launch_floor(int kind, int dir, const double *in, double *out, int n, int nl,
+                        int ld, Tile t, hipStream_t s);
 
 }  // namespace ccfv

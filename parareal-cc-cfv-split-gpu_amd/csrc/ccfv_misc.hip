@@ -240,6 +240,58 @@ hipError_t launch_copy(const double *src, double *dst, size_t n, hipStream_t s)
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// launch-rate floor of the one-field sweeps (instrumentation): the sweeps'
+// grid with no work, and a copy of the field with a sweep's memory pattern
+// and no arithmetic.  One wave per line, lanes of ch contiguous cells.
+// ---------------------------------------------------------------------------
+__global__ void floor_empty_kernel(int *never)
+{
+    if (never && threadIdx.x == 1u << 30) *never = 1;  // no store is ever made
+}
+
+// y pattern: line i is row i (contiguous); per lane ch cells, 16-byte moves
+// (n is a multiple of ch, and even: lanes below n / ch move whole chunks)
+__global__ void floor_rows_kernel(const double *in, double *out, int nl, int n, int ld, int ch)
+{
+    const int line = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (line >= nl || lane * ch >= n) return;
+    const double *src = in + (size_t)line * ld + lane * ch;
+    double *dst = out + (size_t)line * ld + lane * ch;
+    if (ch >= 2) {
+        for (int k = 0; k < ch; k += 2)
+            *reinterpret_cast<double2 *>(dst + k) = *reinterpret_cast<const double2 *>(src + k);
+    } else {
+        dst[0] = src[0];
+    }
+}
+
+// x pattern: line j is column j (stride ld); per lane ch cells of the column
+__global__ void floor_cols_kernel(const double *in, double *out, int nl, int n, int ld, int ch)
+{
+    const int line = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (line >= nl || lane * ch >= n) return;
+    const size_t base = (size_t)(lane * ch) * ld + line;
+    for (int k = 0; k < ch; k++) out[base + (size_t)k * ld] = in[base + (size_t)k * ld];
+}
+
+hipError_t launch_floor(int kind, int dir, const double *in, double *out, int n, int nl,
+                        int ld, Tile t, hipStream_t s)
+{
+    // one wave per line: nl / nw blocks of nw waves (the direct one-field
+    // sweeps' shape, W == NW lines per block)
+    const int nw = t.nw;
+    const dim3 grid((unsigned)((nl + nw - 1) / nw)), block(64 * nw);
+    const int ch = chunk_for(n);
+    if (kind == 0) hipLaunchKernelGGL(floor_empty_kernel, grid, block, 0, s, nullptr);
+    else if (dir == DIR_Y)
+        hipLaunchKernelGGL(floor_rows_kernel, grid, block, 0, s, in, out, nl, n, ld, ch);
+    else hipLaunchKernelGGL(floor_cols_kernel, grid, block, 0, s, in, out, nl, n, ld, ch);
+    return hipGetLastError();
+}
+
 __global__ void nonfinite_kernel(const double *src, size_t n, int *flag)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;

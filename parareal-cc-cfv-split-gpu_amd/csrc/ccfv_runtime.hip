@@ -1157,6 +1157,146 @@ int ccfv_profile_read(ccfv_ctx *c, ccfv_kernel_stat *out)
     return CCFV_OK;
 }
 
+int ccfv_time_sweep(ccfv_ctx *c, int32_t dir, int32_t fused, int32_t nf, double h,
+                    int32_t launches, int32_t layout, int32_t tile_w, int32_t tile_nw,
+                    int32_t graph, double *us)
+{
+    if (!c || !us) return fail(CCFV_EINVAL, "NULL argument");
+    if ((dir != DIR_X && dir != DIR_Y) || (fused && dir != DIR_X) || nf < 1 || nf > 4096 ||
+        launches < 1 || launches > 1 << 20 || !(h > 0.0) || (layout != 0 && layout != 1))
+        return fail(CCFV_EINVAL, "dir 0/1, fused only for x, 1 <= nfields <= 4096, "
+                                 "1 <= launches <= 2^20, h > 0, layout 0/1");
+    const Dir d = (Dir)dir;
+    const SolveTable *tb;
+    int rc = get_table(c, d, h, &tb);
+    if (rc) return rc;
+    const int n = d == DIR_X ? c->P.nx : c->P.ny, nl = d == DIR_X ? c->P.ny : c->P.nx;
+    const Tile t = (tile_w || tile_nw) ? Tile{tile_w, tile_nw} : c->tile[d][nf > 1 ? 1 : 0];
+    if (!tile_ok(t, n, nl, d, post_flag(c) && d == DIR_X, c->P.src_kind != CCFV_SRC_NONE,
+                 c->P.bc))
+        return fail(CCFV_EINVAL, "invalid tile shape for this grid");
+    if (layout && (!tb->line_dev || c->P.nx % PANEL_W || c->P.ny % PANEL_W ||
+                   (d == DIR_X && t.w != PANEL_W)))
+        return fail(CCFV_EINVAL, "the panel layout needs the uniform-displacement sweep, "
+                                 "grids of multiples of 8 and x shapes 8 lines wide");
+    const size_t N = c->ncell;
+    double *scratch = nullptr;
+    HIPCHK(hipMalloc(&scratch, sizeof(double) * N * (size_t)nf));
+    hipError_t e = hipSuccess;
+    for (int f = 0; f < nf && e == hipSuccess; f++)
+        e = launch_copy(c->state, scratch + f * N, N, c->stream);
+    SweepArgs a = make_args(c, d, d == DIR_X ? (fused ? 3 : 1) : 2, *tb, h, 0.0, 0.0, 0, 0.0);
+    a.in = scratch;
+    a.out = scratch;
+    a.in_panel = a.out_panel = layout;
+    a.fstride = (long long)N;
+    if (fused) {
+        a.out2 = scratch;
+        a.field0_2 = 0;
+        a.t0_2 = 0.0;
+        a.jdt2 = h;
+    }
+    const bool post = post_flag(c) && d == DIR_X;
+    hipStream_t keep = c->cur;
+    c->cur = c->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipGraphExec_t gx = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    auto issue = [&]() {
+        hipError_t r = hipSuccess;
+        for (int i = 0; i < launches && r == hipSuccess; i++)
+            r = fused ? run_sweep_fused(c, post, EPI_NONE, a, nf, t)
+                      : run_sweep(c, d, post, EPI_NONE, a, nf, t);
+        return r;
+    };
+    if (e == hipSuccess && graph) {
+        // one eager launch first: lazy per-kernel set-up stays out of the capture
+        e = fused ? run_sweep_fused(c, post, EPI_NONE, a, nf, t)
+                  : run_sweep(c, d, post, EPI_NONE, a, nf, t);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed);
+        if (e == hipSuccess) {
+            hipError_t r = issue();
+            hipGraph_t gr = nullptr;
+            e = hipStreamEndCapture(c->stream, &gr);
+            if (e == hipSuccess) e = r;
+            if (e == hipSuccess) e = hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0);
+            if (gr) (void)hipGraphDestroy(gr);
+        }
+    }
+    // one untimed pass, then the timed one
+    for (int pass = 0; pass < 2 && e == hipSuccess; pass++) {
+        if (pass == 1) e = hipEventRecord(e0, c->stream);
+        if (e == hipSuccess) e = gx ? hipGraphLaunch(gx, c->stream) : issue();
+        if (e == hipSuccess && pass == 1) e = hipEventRecord(e1, c->stream);
+    }
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    c->cur = keep;
+    (void)hipStreamSynchronize(c->stream);
+    if (gx) (void)hipGraphExecDestroy(gx);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(scratch);
+    if (e != hipSuccess) return hip_fail(e, "time sweep");
+    *us = 1e3 * (double)ms / launches;
+    return check_flag(c);
+}
+
+int ccfv_dispatch_floor(ccfv_ctx *c, int32_t dir, int32_t kind, int32_t launches,
+                        int32_t graph, double *us)
+{
+    if (!c || !us) return fail(CCFV_EINVAL, "NULL argument");
+    if ((dir != DIR_X && dir != DIR_Y) || (kind != 0 && kind != 1) || launches < 1 ||
+        launches > 1 << 20)
+        return fail(CCFV_EINVAL, "dir 0/1, kind 0/1, 1 <= launches <= 2^20");
+    const Dir d = (Dir)dir;
+    const int n = d == DIR_X ? c->P.nx : c->P.ny, nl = d == DIR_X ? c->P.ny : c->P.nx;
+    const Tile t = c->tile[d][0];
+    double *scratch = nullptr;
+    HIPCHK(hipMalloc(&scratch, sizeof(double) * c->ncell));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipGraphExec_t gx = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    auto issue = [&]() {
+        hipError_t r = hipSuccess;
+        for (int i = 0; i < launches && r == hipSuccess; i++)
+            r = launch_floor(kind, d, c->state, scratch, n, nl, c->P.ny, t, c->stream);
+        return r;
+    };
+    if (e == hipSuccess && graph) {
+        e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed);
+        if (e == hipSuccess) {
+            hipError_t r = issue();
+            hipGraph_t gr = nullptr;
+            e = hipStreamEndCapture(c->stream, &gr);
+            if (e == hipSuccess) e = r;
+            if (e == hipSuccess) e = hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0);
+            if (gr) (void)hipGraphDestroy(gr);
+        }
+    }
+    // one untimed pass, then the timed one
+    for (int pass = 0; pass < 2 && e == hipSuccess; pass++) {
+        if (pass == 1) e = hipEventRecord(e0, c->stream);
+        if (e == hipSuccess) e = gx ? hipGraphLaunch(gx, c->stream) : issue();
+        if (e == hipSuccess && pass == 1) e = hipEventRecord(e1, c->stream);
+    }
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipStreamSynchronize(c->stream);
+    if (gx) (void)hipGraphExecDestroy(gx);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(scratch);
+    if (e != hipSuccess) return hip_fail(e, "dispatch floor");
+    *us = 1e3 * (double)ms / launches;
+    return CCFV_OK;
+}
+
 int ccfv_sync(ccfv_ctx *c)
 {
     if (!c) return fail(CCFV_EINVAL, "NULL context");

@@ -597,6 +597,66 @@ __device__ __forceinline__ void fast_y_direct(const SweepArgs &a, int field, int
     }
 }
 
+// The same for a field in the panel layout (in and out): row i of the field
+// is 8 contiguous doubles at p * ps + 8 i in every panel p, so a lane's CH
+// cells are CH / 8 runs of 8 (64 bytes) in consecutive panels.
+template <int CH>
+__device__ __forceinline__ void fast_y_direct_panel(const SweepArgs &a, int field, int tile,
+                                                   int W, int NW, double *lds)
+{
+    static_assert(CH >= 8 && PANEL_W == 8, "full-wave lines of whole panels");
+    constexpr bool REG = CH <= 16;
+    constexpr int n = 64 * CH, NP = CH / 8;
+    const int LP = line_pitch(n + fast_ghost_cells(CH));
+    const size_t foff = (size_t)field * (size_t)a.fstride;
+    const size_t ps = (size_t)PANEL_W * (size_t)a.nl;
+    const int i0 = tile * W;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    double *cs = lds + wave * LP;
+    LaneTab<CH, REG> tb;
+    tb.load(a, lane * CH, true);
+    const double mfl = a.tab[4 * n + lane], mbl = a.tab[4 * n + 64 + lane];
+    const double pneg0 = a.tab[2 * n];
+    const double pf = lane == 0 ? pneg0 : tb.M(0, lane * CH);
+    const double pl = lane == 63 ? 0.0 : tb.M(CH - 1, lane * CH + CH - 1);
+    // the lane's first panel
+    const double *in = a.in + foff + (size_t)(lane * NP) * ps;
+    double *out = a.out + foff + (size_t)(lane * NP) * ps;
+    double2 nxt[CH / 2];
+    auto fetch = [&](int row) {
+#pragma unroll
+        for (int q = 0; q < NP; q++) {
+            const double2 *src =
+                reinterpret_cast<const double2 *>(in + (size_t)q * ps + (size_t)PANEL_W * row);
+#pragma unroll
+            for (int k = 0; k < 4; k++) nxt[4 * q + k] = src[k];
+        }
+    };
+    int l = wave;
+    if (l < W) fetch(i0 + l);
+    for (; l < W; l += NW) {
+        const LineK lk = line_k(a, i0 + l);
+        double *own = cs + pidx(CH * lane);
+#pragma unroll
+        for (int k = 0; k < CH / 2; k++) {
+            own[2 * k + (2 * k >> 3)] = nxt[k].x;
+            own[2 * k + 1 + ((2 * k + 1) >> 3)] = nxt[k].y;
+        }
+        if (l + NW < W) fetch(i0 + l + NW);
+        double res[CH];
+        fast_line_chunked<CH, REG, true>(a, cs, lk, lane, tb, mfl, mbl, pf, pl, &res);
+#pragma unroll
+        for (int q = 0; q < NP; q++) {
+            double2 *dst = reinterpret_cast<double2 *>(out + (size_t)q * ps +
+                                                       (size_t)PANEL_W * (i0 + l));
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                dst[k] = make_double2(res[8 * q + 2 * k], res[8 * q + 2 * k + 1]);
+        }
+    }
+}
+
 template <int CH, int LAYOUT = 0>
 __device__ __forceinline__ void fast_y_body(const SweepArgs &a, int field, int tile,
                                            int W, int NW, double *lds, bool direct = false)
@@ -605,6 +665,12 @@ __device__ __forceinline__ void fast_y_body(const SweepArgs &a, int field, int t
     if constexpr (CH >= 8 && LAYOUT == 0) {
         if (direct && a.n == 64 * CH && W == NW) {
             fast_y_direct<CH>(a, field, tile, W, NW, lds);
+            return;
+        }
+    }
+    if constexpr (CH >= 8 && pin && pout) {
+        if (direct && a.n == 64 * CH && W == NW) {
+            fast_y_direct_panel<CH>(a, field, tile, W, NW, lds);
             return;
         }
     }
