@@ -64,6 +64,9 @@ struct ccfv_ctx {
     int iter_cap = 0, iter_valid = 0;
     double *panel = nullptr;
     int panel_fields = 0;
+    // swirl: sin(pi s), cos(pi s) of the faces of x lines (2 (nx + 1)) then
+    // of y lines (2 (ny + 1)) (ensure_trig)
+    double *trig = nullptr;
     int tiles_version = 0;
     std::map<std::string, hipGraphExec_t> gcache;
     std::vector<hipEvent_t> sync_ev;  // cross-stream events (no timing)
@@ -140,6 +143,9 @@ int step_fields(ccfv_ctx *c, const double *in, double *out, int nf, double t0,
 int check_flag(ccfv_ctx *c);
 int validate_problem(const ccfv_problem *p);
 int ensure_para(ccfv_ctx *c, int ns);
+// the face sine / cosine table of a swirl problem (no-op for other
+// velocities), filled on stream s; the caller frees c->trig
+int ensure_trig(ccfv_ctx *c, hipStream_t s);
 void clear_graphs(ccfv_ctx *c);
 double increment_of(const ccfv_ctx *c, int ns, int k, std::vector<double> &slot_u2);
 

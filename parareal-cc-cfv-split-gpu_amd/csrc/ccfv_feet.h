@@ -112,6 +112,61 @@ __host__ __device__ inline double face_foot_swirl(const SwirlLine &l, double lo,
     return 0.5 * (f0 + f1);
 }
 
+// sin(pi (s - delta)) from sp = sin(pi s), cp = cos(pi s) and a small shift:
+// sin(a - e) = sin a cos e - cos a sin e with e = pi delta, and cos e, sin e
+// by their Taylor series, for |e| <= 1/4.  The sums stop at e^12 / 12! and
+// e^11 / 11!; the first terms left out, e^14 / 14! and e^13 / 13!, are below
+// 5e-20 and 3e-18 at |e| = 1/4.  Explicit fma() keeps host and device
+// arithmetic identical.
+__host__ __device__ inline double sin_pi_shift(double sp, double cp, double delta)
+{
+#pragma clang fp contract(off)
+    const double e = M_PI * delta, e2 = e * e;
+    double c = 1.0 / 479001600.0;           // 1/12!
+    c = fma(c, e2, -1.0 / 3628800.0);       // 1/10!
+    c = fma(c, e2, 1.0 / 40320.0);          // 1/8!
+    c = fma(c, e2, -1.0 / 720.0);           // 1/6!
+    c = fma(c, e2, 1.0 / 24.0);             // 1/4!
+    c = fma(c, e2, -0.5);                   // 1/2!
+    c = fma(c, e2, 1.0);
+    double s = -1.0 / 39916800.0;           // 1/11!
+    s = fma(s, e2, 1.0 / 362880.0);         // 1/9!
+    s = fma(s, e2, -1.0 / 5040.0);          // 1/7!
+    s = fma(s, e2, 1.0 / 120.0);            // 1/5!
+    s = fma(s, e2, -1.0 / 6.0);             // 1/3!
+    s = fma(s * e2, e, e);
+    return fma(sp, c, -(cp * s));
+}
+
+// face_foot_swirl with sin(pi s), cos(pi s) of the face from a table (sp,
+// cp: the values sin(M_PI * s) and cos(M_PI * s) give for s = lo + d q) and,
+// when small (pi tau <= 1/4; the swirl speed is at most 1, so the shift
+// tau v of a trace is at most tau), the displaced sines by sin_pi_shift:
+// one table read instead of a sine per face, and the two traced sines at the
+// cost of two short polynomials.  Not bitwise face_foot_swirl: the displaced
+// sines differ by the rounding of the two formulas (a few ulp).
+template <int DIR>
+__host__ __device__ inline double face_foot_swirl_tab(const SwirlLine &l, double lo, double d,
+                                                      double tau, int q, double sp, double cp,
+                                                      bool small)
+{
+#pragma clang fp contract(off)
+    double s = lo + d * q;
+    double ht = 0.5 * tau;
+    auto sq = [](double p) { return DIR == 0 ? p * p : -(p * p); };
+    auto shifted = [&](double delta) {
+        return small ? sin_pi_shift(sp, cp, delta) : sin(M_PI * (s - delta));
+    };
+    const double ps = sq(sp);
+    double v0 = ps * l.a0 * l.chi;
+    double v1 = sq(shifted(tau * v0)) * l.a0 * l.clo;
+    double f0 = s - ht * (v0 + v1);
+    double w0 = ps * l.a1 * l.chi;
+    double w1 = sq(shifted(tau * w0)) * l.a1 * l.clo;
+    double f1 = s - ht * (w0 + w1);
+    return 0.5 * (f0 + f1);
+}
+
 // cell of position x and the offset inside it: floor((x - lo)/d)
 // (T2S4_Parareal.cu:325-328); n bounds the int conversion
 __host__ __device__ inline int cell_of(double lo, double d, int n, double x,

@@ -86,6 +86,13 @@ struct SweepArgs {
     int out_panel;
     double src[5];
     double src_w;
+    // 1 / d: the general sweeps locate feet by multiplying with it
+    double id;
+    // swirl: sin(pi s_q) for the n + 1 faces of a line, then cos(pi s_q)
+    // (null: evaluated per face); trig_small: pi tau <= 1/4, the traced
+    // sines come from the table by angle addition
+    const double *trig;
+    int trig_small;
 };
 
 // columns per panel of the panel layout
@@ -181,6 +188,8 @@ hipError_t launch_fill_exact(const ccfv_problem &p, const ccfv_ic &ic, double t,
 hipError_t launch_copy(const double *src, double *dst, size_t n, hipStream_t s);
 hipError_t launch_nonfinite_check(const double *src, size_t n, int *flag,
                                   hipStream_t s);
+// sin(pi s_q), then cos(pi s_q), for the n + 1 faces s_q = lo + d q
+hipError_t launch_fill_trig(double lo, double d, int n, double *out, hipStream_t s);
 // launch-rate floor (instrumentation): kind 0 an empty kernel, kind 1 a copy
 // in -> out with the one-field sweep's memory pattern of direction dir; one
 // wave per line of n cells, nl lines, row length ld, t.nw waves per block

@@ -94,14 +94,26 @@ template <int DIR>
 __device__ __forceinline__ double face_foot(const SweepArgs &a, const Times &t,
                                             int q, const LineGeo &lg)
 {
-    if (a.vel_kind == CCFV_VEL_SWIRL) return face_foot_swirl<DIR>(lg.sw, a.lo, a.d, a.tau, q);
+    if (a.vel_kind == CCFV_VEL_SWIRL) {
+        if (a.trig)
+            return face_foot_swirl_tab<DIR>(lg.sw, a.lo, a.d, a.tau, q, a.trig[q],
+                                            a.trig[a.n + 1 + q], a.trig_small != 0);
+        return face_foot_swirl<DIR>(lg.sw, a.lo, a.d, a.tau, q);
+    }
     VelParams v{a.vel_kind, a.v0, a.v1, a.v2};
     return face_foot<DIR>(v, a.lo, a.d, a.tau, t.hi, t.lo, q, lg.o0, lg.o1);
 }
 
+// cell_of with the division by d as a multiplication by 1/d (equal when d
+// is a power of two)
 __device__ __forceinline__ int cell_of(const SweepArgs &a, double x, double &xi)
 {
-    return cell_of(a.lo, a.d, a.n, x, xi);
+#pragma clang fp contract(off)
+    double v = floor((x - a.lo) * a.id);
+    v = fmin(fmax(v, -4.0 * a.n), 4.0 * a.n);
+    const int m = (int)v;
+    xi = (x - (a.lo + a.d * m)) * a.id;
+    return m;
 }
 
 // ---------------------------------------------------------------------------
@@ -175,10 +187,14 @@ __device__ __forceinline__ double deriv(const double w[5], double xi,
                      (2.0 * x3 - 3.0 * xi) * w[4]);
 }
 
+// k mod n in [0, n); feet within one period of the line need no division
 __device__ __forceinline__ int wrapn(int k, int n)
 {
-    k %= n;
-    return k < 0 ? k + n : k;
+    if (k < -n || k >= 2 * n) {
+        k %= n;
+        return k < 0 ? k + n : k;
+    }
+    return k < 0 ? k + n : (k >= n ? k - n : k);
 }
 
 struct Face {
@@ -265,7 +281,7 @@ __device__ __forceinline__ double deriv_ref(const SweepArgs &a,
     double xi;
     int m = cell_of(a, xd, xi);
     int md = clampi(m, 0, a.n - 1);
-    if (md != m) xi = (xd - (a.lo + a.d * md)) / a.d;
+    if (md != m) xi = (xd - (a.lo + a.d * md)) * a.id;
     double w[5];
     window<BC_REF>(src, a.n, g, md, w);
     if (a.quirk) {  // :476-479 — iL==Nx-2 takes CNp1, iL==1 takes Cm2
@@ -311,7 +327,7 @@ __device__ __forceinline__ Face eval_face(const SweepArgs &a, const Times &tt,
         int m = clampi(cell_of(a, xl, xi), 0, n - 1);
         {
 #pragma clang fp contract(off)
-            xi = (xl - (a.lo + a.d * m)) / a.d;
+            xi = (xl - (a.lo + a.d * m)) * a.id;
         }
         double w[5];
         window<BC_REF>(cs, n, g, m, w);
@@ -328,7 +344,7 @@ __device__ __forceinline__ Face eval_face(const SweepArgs &a, const Times &tt,
             int m2 = clampi(cell_of(a, xr, xi2), 0, n - 1);
             {
 #pragma clang fp contract(off)
-                xi2 = (xr - (a.lo + a.d * m2)) / a.d;
+                xi2 = (xr - (a.lo + a.d * m2)) * a.id;
             }
             double w2[5];
             window<BC_REF>(cs, n, g, m2, w2);
@@ -362,8 +378,7 @@ __device__ __forceinline__ double remap_cell(const SweepArgs &a,
             return 0.0;
         }
         if (mR == mL) return R.ivR - L.ivL;
-        int w = (mL + 1) % n;
-        w = w < 0 ? w + n : w;
+        int w = wrapn(mL + 1, n);
         double s = 0.0;
         for (int m = mL + 1; m <= mR - 1; m++) {
             s += cs[pidx(w)];

@@ -240,6 +240,25 @@ hipError_t launch_copy(const double *src, double *dst, size_t n, hipStream_t s)
     return hipGetLastError();
 }
 
+// face sines and cosines of the swirl, with the expressions of
+// face_foot_swirl (so the table holds the values it would compute)
+__global__ void fill_trig_kernel(double lo, double d, int n, double *out)
+{
+#pragma clang fp contract(off)
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > n) return;
+    const double s = lo + d * q;
+    out[q] = sin(M_PI * s);
+    out[n + 1 + q] = cos(M_PI * s);
+}
+
+hipError_t launch_fill_trig(double lo, double d, int n, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(fill_trig_kernel, dim3((n + 1 + 255) / 256), dim3(256), 0, s, lo, d, n,
+                       out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // launch-rate floor of the one-field sweeps (instrumentation): the sweeps'
 // grid with no work, and a copy of the field with a sweep's memory pattern

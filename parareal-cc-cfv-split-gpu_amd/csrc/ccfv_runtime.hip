@@ -286,6 +286,11 @@ SweepArgs make_args(ccfv_ctx *c, Dir dir, int substep, const SolveTable &T,
     a.tab = T.dev;
     a.lineS = T.line_dev;
     a.lineK = T.line_dev ? T.line_dev + a.nl : nullptr;
+    a.id = 1.0 / a.d;
+    if (c->trig) {
+        a.trig = dir == DIR_X ? c->trig : c->trig + 2 * (size_t)(P.nx + 1);
+        a.trig_small = M_PI * std::fabs(a.tau) <= 0.25 ? 1 : 0;
+    }
     a.vel_kind = P.vel_kind;
     a.v0 = P.vel[0];
     a.v1 = P.vel[1];
@@ -631,6 +636,22 @@ void free_para(ccfv_ctx *c)
     c->para_ns = 0;
 }
 
+int ensure_trig(ccfv_ctx *c, hipStream_t s)
+{
+    if (c->trig || c->P.vel_kind != CCFV_VEL_SWIRL) return CCFV_OK;
+    const ccfv_problem &P = c->P;
+    const size_t nb = 2 * (size_t)(P.nx + 1) + 2 * (size_t)(P.ny + 1);
+    if (hipMalloc(&c->trig, sizeof(double) * nb) != hipSuccess) {
+        (void)hipGetLastError();
+        c->trig = nullptr;
+        return fail(CCFV_ENOMEM, "swirl face table");
+    }
+    HIPCHK(launch_fill_trig(P.x_lo, cell_width(P, DIR_X), P.nx, c->trig, s));
+    HIPCHK(launch_fill_trig(P.y_lo, cell_width(P, DIR_Y), P.ny,
+                            c->trig + 2 * (size_t)(P.nx + 1), s));
+    return CCFV_OK;
+}
+
 int ensure_para(ccfv_ctx *c, int ns)
 {
     if (c->para_ns == ns && c->chain_blocks == c->P.ny / c->chain_tile.w)
@@ -766,6 +787,10 @@ int ccfv_create(const ccfv_problem *p, ccfv_ctx **out)
     }
     *c->h_flag = 0;
     c->cur = c->stream;
+    if (int rc2 = ensure_trig(c, c->stream)) {
+        ccfv_destroy(c);
+        return rc2;
+    }
     // stream names in rocprofv3 traces
     roctxNameHipStream("ccfv chain", c->stream);
     roctxNameHipStream("ccfv fine", c->stream2);
@@ -802,6 +827,7 @@ void ccfv_destroy(ccfv_ctx *c)
         (void)hipFree(kv.second.line_dev);
     }
     (void)hipFree(c->state);
+    (void)hipFree(c->trig);
     (void)hipFree(c->d_flag);
     if (c->h_flag) (void)hipHostFree(c->h_flag);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

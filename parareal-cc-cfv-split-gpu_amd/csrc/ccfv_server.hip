@@ -135,6 +135,7 @@ int place(ccfv_server *s, Request &q)
     if ((rc = job_setup(q.job, &c, q.r.t0, q.r.t1, q.r.nt_fine, ns, q.r.kmax, q.r.tol)))
         return rc;
     }
+    if ((rc = ensure_trig(&c, s->run.stream))) return rc;
     HIPCHK(hipMemsetAsync(c.d_flag, 0, sizeof(int), s->run.stream));
     if (!q.u0.empty()) {
         HIPCHK(hipMemcpyAsync(c.U, q.u0.data(), f, hipMemcpyHostToDevice, s->run.stream));
@@ -151,6 +152,8 @@ void release_tables(ccfv_ctx &c)
         (void)hipFree(kv.second.line_dev);
     }
     c.tables.clear();
+    (void)hipFree(c.trig);
+    c.trig = nullptr;
 }
 
 // finished: result to the host, arena block and slot back
