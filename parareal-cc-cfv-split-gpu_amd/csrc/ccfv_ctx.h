@@ -67,6 +67,12 @@ struct ccfv_ctx {
     // swirl: sin(pi s), cos(pi s) of the faces of x lines (2 (nx + 1)) then
     // of y lines (2 (ny + 1)) (ensure_trig)
     double *trig = nullptr;
+    // tables placed by the caller for one step size (the allocation-free
+    // entry point: its tables live in static device memory); with no_alloc
+    // set, get_table fails rather than allocate a table for another step
+    const ccfv::SolveTable *preset[2] = {nullptr, nullptr};
+    double preset_h = 0.0;
+    bool no_alloc = false;
     int tiles_version = 0;
     std::map<std::string, hipGraphExec_t> gcache;
     std::vector<hipEvent_t> sync_ev;  // cross-stream events (no timing)
@@ -109,6 +115,10 @@ struct TraceRange {
 };
 // launch shape of the coarse chain's x launches (ccfv_ctx::chain_tile)
 Tile chain_tile_for(const ccfv_problem &p, Tile fallback);
+// default launch shape of one sweep direction (one field or a batch)
+Tile pick_tile(Dir dir, int n, int nl, bool batch, bool post, bool src);
+// lane-scan doubling steps of a table (its mneg and pneg parts)
+void scan_steps(const double *tab, int n, int &sf, int &sb);
 double cell_width(const ccfv_problem &P, Dir dir);
 int get_table(ccfv_ctx *c, Dir dir, double h, const SolveTable **out);
 SweepArgs make_args(ccfv_ctx *c, Dir dir, int substep, const SolveTable &T,

@@ -197,6 +197,7 @@ __host__ __device__ inline double corner(double olo, double od, int L)
 __host__ __device__ inline void fir_weights(double xi, double coef, double inv12d,
                                             double kap[5])
 {
+#pragma clang fp contract(off)
     const double x2 = xi * xi, x3 = x2 * xi;
     const double cl[5] = {-1.0, 7.0, 7.0, -1.0, 0.0};
     const double dc[5] = {4.0, -26.0, 30.0, -10.0, 2.0};

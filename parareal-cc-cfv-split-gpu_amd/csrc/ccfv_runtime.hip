@@ -6,6 +6,7 @@
 // through ccfv_set_state / ccfv_get_state.
 #include "ccfv_ctx.h"
 #include "ccfv_feet.h"
+#include "ccfv_tables.h"
 
 #include <algorithm>
 #include <cmath>
@@ -95,154 +96,97 @@ double cell_width(const ccfv_problem &P, Dir dir)
     return dir == DIR_X ? (P.x_hi - P.x_lo) / P.nx : (P.y_hi - P.y_lo) / P.ny;
 }
 
+// lane-scan doubling steps of a table: after s steps a lane's carry
+// neglects the product of the factors over 2^s whole chunks; stop once that
+// is below 1e-18 (far under FP64 rounding of the carried value)
+void scan_steps(const double *tab, int n, int &sf, int &sb)
+{
+    const int ch = chunk_for(n), nl = n / ch;
+    double pf = 0.0, pb = 0.0;
+    for (int l = 0; l < nl; l++) {
+        double f = 1.0, b = 1.0;
+        for (int k = 0; k < ch; k++) {
+            f *= std::fabs(tab[l * ch + k]);
+            b *= std::fabs(tab[2 * n + l * ch + k]);
+        }
+        pf = std::max(pf, f);
+        pb = std::max(pb, b);
+    }
+    auto steps = [nl](double p) {
+        int full = 0;
+        while ((1 << full) < nl) full++;
+        if (full < 1) full = 1;
+        if (p <= 0.0) return 1;
+        if (p >= 1.0) return full;
+        double need = std::log(1e-18) / std::log(p);  // chunks to span
+        int s = 1;
+        while ((double)(1 << s) < need && s < full) s++;
+        return std::min(s, full);
+    };
+    sf = steps(pf);
+    sb = steps(pb);
+}
+
+// the scalars of a table (thomas_fill) and, in tab / lines, its arrays on
+// the host; lines: 7 nl doubles (shift per line, then 6 weights per line),
+// filled and *fast set when every line takes the uniform-offset sweep
+void table_host(const ccfv_problem &P, Dir dir, double h, SolveTable &T,
+                std::vector<double> &tab, std::vector<double> &lines, bool *fast)
+{
+    const int n = dir == DIR_X ? P.nx : P.ny;
+    const int nl = dir == DIR_X ? P.ny : P.nx;
+    std::vector<double> work(2 * (size_t)n);
+    tab.assign(4 * (size_t)n + 128, 1.0);
+    const TabScalars S =
+        thomas_fill(n, cell_width(P, dir), dir == DIR_X ? 0.25 : 0.5, P.nu, h,
+                    P.bc == CCFV_BC_PERIODIC, chunk_for(n), tab.data(), work.data());
+    T.ca = S.ca;
+    T.cb = S.cb;
+    T.coef = S.coef;
+    T.vl = S.vl;
+    T.diag0 = S.diag0;
+    T.smden = S.smden;
+    scan_steps(tab.data(), n, T.sf, T.sb);
+    *fast = false;
+    lines.clear();
+    if (!uniform_problem(P) || getenv("CCFV_NO_FAST")) return;
+    lines.assign(7 * (size_t)nl, 0.0);
+    for (int L = 0; L < nl; L++)
+        if (!line_fill(P, (int)dir, h, T.coef, L, &lines[L], &lines[nl + 6 * (size_t)L])) {
+            lines.clear();
+            return;
+        }
+    *fast = true;
+}
+
 // Thomas factors of the constant tridiagonal matrix of one sweep direction
-// and step size (T2S4_Parareal.cu:550-561, :583-594, :666-673).  theta is
-// 1/4 for the x half-steps and 1/2 for the y step.
+// and step size, and for the uniform-offset sweep the per-line shift and
+// filter weights (thomas_fill, line_fill), in device memory of the context
 int get_table(ccfv_ctx *c, Dir dir, double h, const SolveTable **out)
 {
+    if (c->preset[dir] && h == c->preset_h) {
+        *out = c->preset[dir];
+        return CCFV_OK;
+    }
+    if (c->no_alloc) return fail(CCFV_EINVAL, "no table for this step size");
     auto key = std::make_pair((int)dir, h);
     auto it = c->tables.find(key);
     if (it != c->tables.end()) {
         *out = &it->second;
         return CCFV_OK;
     }
-    const ccfv_problem &P = c->P;
-    const int n = dir == DIR_X ? P.nx : P.ny;
-    const double d = cell_width(P, dir);
-    const double theta = dir == DIR_X ? 0.25 : 0.5;
-    const double nu = P.nu;
-    const double r2 = h / (d * d);
     SolveTable T;
-    T.ca = 1.0 / 12 - theta * nu * r2;
-    T.cb = 10.0 / 12 + 2.0 * theta * nu * r2;
-    T.coef = theta * nu * (h / d);
-    std::vector<double> diag(n, T.cb), p(n), den(n), mneg(n), pneg(n), z(n, 0.0);
-    const bool per = P.bc == CCFV_BC_PERIODIC;
-    if (per) {
-        // A = T + u v^T, gamma = -b, u = (gamma,0..0,a), v = (1,0..0,a/gamma)
-        double gamma = -T.cb;
-        T.vl = T.ca / gamma;
-        diag[0] = T.cb - gamma;
-        diag[n - 1] = T.cb - T.ca * T.vl;
-    }
-    T.diag0 = diag[0];
-    p[0] = T.ca / diag[0];
-    den[0] = 1.0 / diag[0];
-    for (int i = 1; i < n; i++) {
-        den[i] = 1.0 / (diag[i] - p[i - 1] * T.ca);
-        p[i] = T.ca * den[i];
-    }
-    mneg[0] = 0.0;
-    for (int i = 1; i < n; i++) mneg[i] = -T.ca * den[i];
-    for (int i = 0; i < n; i++) pneg[i] = -p[i];
-    pneg[n - 1] = 0.0;
-    if (per) {
-        // z = T^{-1} u by the same recurrences
-        std::vector<double> u(n, 0.0), q(n);
-        u[0] = -T.cb;
-        u[n - 1] = T.ca;
-        q[0] = u[0] / diag[0];
-        for (int i = 1; i < n; i++) q[i] = (u[i] - q[i - 1] * T.ca) * den[i];
-        z[n - 1] = q[n - 1];
-        for (int i = n - 2; i >= 0; i--) z[i] = q[i] - p[i] * z[i + 1];
-        T.smden = 1.0 + (z[0] + T.vl * z[n - 1]);
-    }
-    // Doubling steps of the lane scans: after s steps a lane's carry neglects
-    // the product of the factors over 2^s whole chunks; stop once that is
-    // below 1e-18 (far under FP64 rounding of the carried value).
-    {
-        const int ch = chunk_for(n), nl = n / ch;
-        double pf = 0.0, pb = 0.0;
-        for (int l = 0; l < nl; l++) {
-            double f = 1.0, b = 1.0;
-            for (int k = 0; k < ch; k++) {
-                f *= std::fabs(mneg[l * ch + k]);
-                b *= std::fabs(pneg[l * ch + k]);
-            }
-            pf = std::max(pf, f);
-            pb = std::max(pb, b);
-        }
-        auto steps = [nl](double p) {
-            int full = 0;
-            while ((1 << full) < nl) full++;
-            if (full < 1) full = 1;
-            if (p <= 0.0) return 1;
-            if (p >= 1.0) return full;
-            double need = std::log(1e-18) / std::log(p);  // chunks to span
-            int s = 1;
-            while ((double)(1 << s) < need && s < full) s++;
-            return std::min(s, full);
-        };
-        T.sf = steps(pf);
-        T.sb = steps(pb);
-    }
-    std::vector<double> tab(4 * (size_t)n + 128, 1.0);
-    std::copy(mneg.begin(), mneg.end(), tab.begin());
-    // the device solves take the compact operator's 1/12 with the pivots:
-    // r = b_{i-1} + 10 b_i + b_{i+1}, first forward value r * (den / 12)
-    for (int i = 0; i < n; i++) tab[n + i] = den[i] / 12.0;
-    std::copy(pneg.begin(), pneg.end(), tab.begin() + 2 * n);
-    std::copy(z.begin(), z.end(), tab.begin() + 3 * n);
-    {
-        // chunk products in the order the lanes fold them; 1 on idle lanes
-        const int ch = chunk_for(n), nl = n / ch;
-        for (int l = 0; l < nl; l++) {
-            double f = 1.0, b = 1.0;
-            for (int k = 0; k < ch; k++) f = mneg[l * ch + k] * f;
-            for (int k = ch - 1; k >= 0; k--) b = pneg[l * ch + k] * b;
-            tab[4 * (size_t)n + l] = f;
-            tab[4 * (size_t)n + 64 + l] = b;
-        }
-    }
+    std::vector<double> tab, ld;
+    bool fast = false;
+    table_host(c->P, dir, h, T, tab, ld, &fast);
     HIPCHK(hipMalloc(&T.dev, sizeof(double) * tab.size()));
     HIPCHK(hipMemcpy(T.dev, tab.data(), sizeof(double) * tab.size(),
                      hipMemcpyHostToDevice));
-    // uniform-displacement path: locate face 0 of every line exactly as the
-    // general path does; accept the sweep when every xi is >= 1e-9 from a
-    // cell edge (then every face's floor() agrees with face 0's)
-    // (a source term takes the general path)
-    const bool uniform = per && P.src_kind == CCFV_SRC_NONE &&
-                         (P.vel_kind == CCFV_VEL_SOLID_BODY || P.vel_kind == CCFV_VEL_CONSTANT);
-    if (uniform && !getenv("CCFV_NO_FAST")) {
-        const int nl = dir == DIR_X ? P.ny : P.nx;
-        const double lo = dir == DIR_X ? P.x_lo : P.y_lo;
-        const double olo = dir == DIR_X ? P.y_lo : P.x_lo;
-        const double od = cell_width(P, dir == DIR_X ? DIR_Y : DIR_X);
-        const double tau = dir == DIR_X ? 0.5 * h : h;
-        const double inv12d = 1.0 / (12.0 * d);
-        VelParams v{P.vel_kind, P.vel[0], P.vel[1], P.vel[2]};
-        std::vector<double> ld(7 * (size_t)nl);
-        bool ok = true;
-        for (int L = 0; L < nl && ok; L++) {
-            double o0 = corner(olo, od, L), o1 = corner(olo, od, L + 1);
-            double xb = dir == DIR_X
-                            ? face_foot<0>(v, lo, d, tau, 0.0, 0.0, 0, o0, o1)
-                            : face_foot<1>(v, lo, d, tau, 0.0, 0.0, 0, o0, o1);
-            double xi;
-            int m = cell_of(lo, d, n, xb, xi);
-            if (!(xi > 1e-9 && xi < 1.0 - 1e-9) || std::abs(m) > 2 * n) {
-                ok = false;
-                break;
-            }
-            ld[L] = (double)m;
-            // b_i = C_m + K_{m+1} - K_m with K_m = sum_s kap_s C_{m-2+s}
-            // (m = i + shift) as one 6-tap filter over C_{m-2} .. C_{m+3}
-            double kap[5];
-            fir_weights(xi, T.coef, inv12d, kap);
-            double *w = &ld[nl + 6 * (size_t)L];
-            w[0] = -kap[0];
-            w[1] = kap[0] - kap[1];
-            w[2] = kap[1] - kap[2] + 1.0;
-            w[3] = kap[2] - kap[3];
-            w[4] = kap[3] - kap[4];
-            w[5] = kap[4];
-        }
-        if (ok) {
-            HIPCHK(hipMalloc(&T.line_dev, sizeof(double) * ld.size()));
-            HIPCHK(hipMemcpy(T.line_dev, ld.data(), sizeof(double) * ld.size(),
-                             hipMemcpyHostToDevice));
-            T.fast_ok = true;
-        }
+    if (fast) {
+        HIPCHK(hipMalloc(&T.line_dev, sizeof(double) * ld.size()));
+        HIPCHK(hipMemcpy(T.line_dev, ld.data(), sizeof(double) * ld.size(),
+                         hipMemcpyHostToDevice));
+        T.fast_ok = true;
     }
     auto res = c->tables.emplace(key, T);
     *out = &res.first->second;
