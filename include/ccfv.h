@@ -204,6 +204,22 @@ int ccfv_coarse(ccfv_ctx *ctx, double tn, double DT);
 int ccfv_fine(ccfv_ctx *ctx, double tn, double DT, int32_t J);
 /* Plain serial-in-time solve: nt steps of (t1-t0)/nt from t0 (:1224-1243). */
 int ccfv_plain_solve(ccfv_ctx *ctx, double t0, double t1, int32_t nt);
+/* The plain serial-in-time solve of ccfv_plain_solve (nt steps of
+ * (t1-t0)/nt from t0, in place) on a caller's device field u of nx*ny
+ * doubles, queued on the caller's stream (NULL: the default stream), that
+ * allocates no device memory.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under stream capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Everything the sweeps need
+ * beyond u (the crossed-feet flag, both directions' solve tables, the
+ * uniform-offset sweep's per-line tables, the swirl's face table) lives in
+ * one static 512 KiB device array per GPU, built there by kernels: grids up
+ * to 2048 x 2048 fit, larger ones are refused with CCFV_EINVAL.  Calls that
+ * overlap on one GPU race on that array.  Results are bitwise those of
+ * ccfv_plain_solve.  Blocks until the solve is done (it reads the flag);
+ * returns CCFV_ECFL when feet crossed, as ccfv_plain_solve. */
+int ccfv_plain_solve_noalloc(const ccfv_problem *p, double *u, double t0, double t1,
+                             int32_t nt, void *stream);
+
 /* Steps n = n_begin .. n_end-1 of the serial fine loop, step n from
  * t_origin + n*dt with step size dt, as the reference's loop computes them
  * (:1234-1236); lets a caller stop every T_span steps for output. */

@@ -58,6 +58,9 @@ _SIGS = {
                                  ctypes.c_double, ctypes.c_int32]),
     "ccfv_plain_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_int32]),
+    "ccfv_plain_solve_noalloc": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.c_void_p,
+                                                ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_int32, ctypes.c_void_p]),
     "ccfv_advance": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_int32, ctypes.c_int32]),
     "ccfv_parareal_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double,
@@ -164,6 +167,15 @@ def device_count() -> int:
 
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(_dp)
+
+
+def plain_solve_noalloc(problem: Problem, u_dev: int, t0: float, t1: float, nt: int,
+                        stream: int = 0):
+    """ccfv_plain_solve_noalloc on a device field at address u_dev (nx*ny
+    doubles), on a HIP stream (0: the default stream)."""
+    _check(load().ccfv_plain_solve_noalloc(ctypes.byref(problem), ctypes.c_void_p(u_dev),
+                                           t0, t1, nt, ctypes.c_void_p(stream or None)),
+           "ccfv_plain_solve_noalloc")
 
 
 class Context:
@@ -505,4 +517,4 @@ class Server:
 
 __all__ = ["Context", "Batch", "Server", "CcfvError", "load", "device_count",
            "types", "Problem", "IC", "ParaReport", "exported_symbols",
-           "request_bytes"]
+           "request_bytes", "plain_solve_noalloc"]
