@@ -406,6 +406,23 @@ int  ccfv_batch_parareal(ccfv_batch *b, const double *t0, const double *t1,
  * of ccfv_batch_parareal that reach kmax above tol finish with their serial
  * fine solve, in the same lock-step rounds as the others). */
 int  ccfv_batch_set_option(ccfv_batch *b, int32_t key, int64_t value);
+/* For the batch, CCFV_OPT_GRAPHS = 1 also makes its phases of ragged
+ * launches hipGraphs: a phase whose launch list has been seen before (same
+ * kernels, shapes, item counts and descriptor offsets) is captured once and
+ * replayed with one graph launch; the descriptors are uploaded before each
+ * replay.  Results are the same either way.
+ *
+ * Launch shapes of the batch's sweeps, tuned per line class (direction,
+ * cells per line, sweep path, boundary mode, source): for each class the
+ * member with the most lines is timed (ccfv_autotune's timing, step size
+ * (t1-t0)/nt of that member, batch_fields fields for the batch shape) and
+ * every member of the class takes the fastest shapes, so members of one
+ * class keep sharing one launch per stage.  The coarse chain's shape is not
+ * changed (the increments' block sums depend on it): each problem's results
+ * stay bitwise those it gets alone.  (ccfv_batch_create gives every member
+ * fixed default shapes.) */
+int  ccfv_batch_autotune(ccfv_batch *b, const double *t0, const double *t1,
+                         const int32_t *nt, int32_t batch_fields);
 
 /* ---- serving engine ------------------------------------------------------
  * (no counterpart in the reference, which solves one problem per process.)

@@ -104,6 +104,8 @@ _SIGS = {
                                            ctypes.POINTER(ParaReport)]),
     "ccfv_batch_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
                                              ctypes.c_int64]),
+    "ccfv_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp,
+                                           ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
     "ccfv_request_bytes": (ctypes.c_int, [ctypes.POINTER(types.Request),
                                           ctypes.POINTER(ctypes.c_uint64)]),
     "ccfv_server_create": (ctypes.c_int, [ctypes.POINTER(types.ServerConfig),
@@ -413,6 +415,16 @@ class Batch:
     def set_option(self, key: int, value: int):
         """ccfv_set_option on every member (e.g. OPT_SERIAL_FINISH)."""
         _check(load().ccfv_batch_set_option(self._h, key, value), "ccfv_batch_set_option")
+
+    def autotune(self, t0, t1, nt, batch_fields: int = 16):
+        """Launch shapes per line class (ccfv_batch_autotune)."""
+        n = len(self)
+        a0 = _per_problem(t0, n, np.float64)
+        a1 = _per_problem(t1, n, np.float64)
+        an = _per_problem(nt, n, np.int32)
+        _check(load().ccfv_batch_autotune(self._h, _ptr(a0), _ptr(a1),
+                                          an.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                          batch_fields), "ccfv_batch_autotune")
 
     def plain(self, t0, t1, nt):
         n = len(self)

@@ -5,6 +5,7 @@
 
 #include <functional>
 #include <map>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -54,6 +55,11 @@ struct Phase {
 // Issues phases on one stream.  The descriptors go through two pinned host +
 // device arenas used alternately, so building phase p+1 on the host overlaps
 // the device running phase p.
+// With graphs on, the launches of a phase (not its descriptor upload) are
+// one hipGraph: a launch list seen a second time — same kernels, shapes,
+// item counts and descriptor offsets in the same arena — is captured and
+// from then on replayed with one graph launch; the kernels read the new
+// descriptors from the arena, so only the launch list has to repeat.
 struct PhaseRunner {
     hipStream_t stream = nullptr;
     char *hbuf[2] = {nullptr, nullptr};
@@ -61,9 +67,15 @@ struct PhaseRunner {
     size_t cap[2] = {0, 0};
     hipEvent_t done[2] = {nullptr, nullptr};
     int slot = 0;
+    bool graphs = false;
+    std::map<std::string, hipGraphExec_t> cache;
+    std::map<std::string, int> seen;
+    long long graph_launches = 0, eager_launches = 0;  // phases issued each way
     int init();
     void release();
+    void clear_cache();
     int run(Phase &ph);
+    int issue(const Phase &ph, const char *d);
 };
 
 // A run of chained Strang steps of one problem (as run_steps), emitted into

@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <tuple>
 
 using namespace ccfv;
 using namespace ccfv::rt;
@@ -233,6 +235,77 @@ int ccfv_batch_set_option(ccfv_batch *b, int32_t key, int64_t value)
         int rc = ccfv_set_option(c, key, value);
         if (rc) return rc;
     }
+    if (key == CCFV_OPT_GRAPHS) {
+        HIPCHK(hipStreamSynchronize(b->run.stream));
+        b->run.graphs = value != 0;
+        b->run.clear_cache();
+    }
+    return CCFV_OK;
+}
+
+int ccfv_batch_autotune(ccfv_batch *b, const double *t0, const double *t1, const int32_t *nt,
+                        int32_t batch_fields)
+{
+    if (!b || !t0 || !t1 || !nt) return fail(CCFV_EINVAL, "NULL argument");
+    if (batch_fields < 1 || batch_fields > 4096)
+        return fail(CCFV_EINVAL, "batch_fields must be in [1, 4096]");
+    int rc;
+    if ((rc = join_contexts(b))) return rc;
+    const int np = (int)b->ctx.size();
+    // a line class: direction, cells per line, sweep path, boundary mode,
+    // source; its representative is the member with the most lines
+    struct Class {
+        int rep = -1, lines = 0;
+        Tile single{0, 0}, batch{0, 0};
+    };
+    std::map<std::tuple<int, int, int, int, int>, Class> classes;
+    auto key_of = [&](ccfv_ctx *c, Dir d, double h) {
+        const SolveTable *t = nullptr;
+        int r = get_table(c, d, h, &t);
+        const int fast = r == CCFV_OK && t->line_dev ? 1 : 0;
+        return std::make_tuple((int)d, d == DIR_X ? c->P.nx : c->P.ny, fast, c->P.bc,
+                               c->P.src_kind != CCFV_SRC_NONE ? 1 : 0);
+    };
+    std::vector<double> hs(np);
+    for (int i = 0; i < np; i++) {
+        if (nt[i] < 1 || !(t1[i] > t0[i])) return fail(CCFV_EINVAL, "bad time interval");
+        hs[i] = (t1[i] - t0[i]) / nt[i];
+        for (int d = 0; d < 2; d++) {
+            ccfv_ctx *c = b->ctx[i];
+            Class &k = classes[key_of(c, (Dir)d, hs[i])];
+            const int lines = d == 0 ? c->P.ny : c->P.nx;
+            if (lines > k.lines) {
+                k.lines = lines;
+                k.rep = i;
+            }
+        }
+    }
+    for (auto &kv : classes) {
+        Class &k = kv.second;
+        const Dir d = (Dir)std::get<0>(kv.first);
+        if ((rc = autotune_dir(b->ctx[k.rep], d, batch_fields, hs[k.rep]))) return rc;
+        k.single = b->ctx[k.rep]->tile[d][0];
+        k.batch = b->ctx[k.rep]->tile[d][1];
+    }
+    // every member of a class takes its shapes (when its grid admits them),
+    // so the members still share one launch per stage and class; the coarse
+    // chain's shape stays fixed (the increments' block sums depend on it)
+    for (int i = 0; i < np; i++) {
+        ccfv_ctx *c = b->ctx[i];
+        const bool post = post_flag(c), src = c->P.src_kind != CCFV_SRC_NONE;
+        for (int d = 0; d < 2; d++) {
+            const Class &k = classes[key_of(c, (Dir)d, hs[i])];
+            const int n = d == 0 ? c->P.nx : c->P.ny, nl = d == 0 ? c->P.ny : c->P.nx;
+            if (tile_ok(k.single, n, nl, (Dir)d, post && d == 0, src, c->P.bc))
+                c->tile[d][0] = k.single;
+            if (tile_ok(k.batch, n, nl, (Dir)d, post && d == 0, src, c->P.bc))
+                c->tile[d][1] = k.batch;
+        }
+        c->tiles_version++;
+        clear_graphs(c);
+    }
+    HIPCHK(hipStreamSynchronize(b->run.stream));
+    b->run.clear_cache();
     return CCFV_OK;
 }
 

@@ -117,6 +117,10 @@ struct TraceRange {
 Tile chain_tile_for(const ccfv_problem &p, Tile fallback);
 // default launch shape of one sweep direction (one field or a batch)
 Tile pick_tile(Dir dir, int n, int nl, bool batch, bool post, bool src);
+// times every valid launch shape of direction d for one field and for
+// batch_fields fields at step size h (copies of the state in scratch) and
+// keeps the fastest in c->tile[d][0] and c->tile[d][1]
+int autotune_dir(ccfv_ctx *c, Dir d, int batch_fields, double h);
 // lane-scan doubling steps of a table (its mneg and pneg parts)
 void scan_steps(const double *tab, int n, int &sf, int &sb);
 double cell_width(const ccfv_problem &P, Dir dir);

@@ -161,9 +161,17 @@ int PhaseRunner::init()
     return CCFV_OK;
 }
 
+void PhaseRunner::clear_cache()
+{
+    for (auto &kv : cache) (void)hipGraphExecDestroy(kv.second);
+    cache.clear();
+    seen.clear();
+}
+
 void PhaseRunner::release()
 {
     if (stream) (void)hipStreamSynchronize(stream);
+    clear_cache();
     for (int s = 0; s < 2; s++) {
         if (hbuf[s]) (void)hipHostFree(hbuf[s]);
         if (dbuf[s]) (void)hipFree(dbuf[s]);
@@ -185,6 +193,9 @@ int PhaseRunner::run(Phase &ph)
     // this arena was last read by the phase before the previous one
     HIPCHK(hipEventSynchronize(done[s]));
     if (bytes > cap[s]) {
+        // captured phases hold addresses in the arena that goes away
+        HIPCHK(hipStreamSynchronize(stream));
+        clear_cache();
         if (hbuf[s]) (void)hipHostFree(hbuf[s]);
         if (dbuf[s]) (void)hipFree(dbuf[s]);
         hbuf[s] = nullptr;
@@ -198,6 +209,45 @@ int PhaseRunner::run(Phase &ph)
     memcpy(hbuf[s], ph.host.data(), bytes);
     HIPCHK(hipMemcpyAsync(dbuf[s], hbuf[s], bytes, hipMemcpyHostToDevice, stream));
     const char *d = dbuf[s];
+    if (graphs) {
+        // the launch list and the arena it reads
+        std::string key((const char *)&d, sizeof(d));
+        for (const Launch &L : ph.launches) {
+            const long long f[] = {L.kind, L.dir, L.tile.w, L.tile.nw, L.ch, L.bc, L.fast,
+                                   L.post, L.fuse, L.epi, L.nitems, L.nblocks,
+                                   (long long)L.lds, (long long)L.items, (long long)L.prefix};
+            key.append((const char *)f, sizeof(f));
+        }
+        auto it = cache.find(key);
+        if (it == cache.end() && ++seen[key] >= 2) {
+            if (cache.size() >= 256) clear_cache();
+            HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+            int r2 = issue(ph, d);
+            hipGraph_t gr = nullptr;
+            hipError_t e = hipStreamEndCapture(stream, &gr);
+            hipGraphExec_t x = nullptr;
+            if (r2 == CCFV_OK && e == hipSuccess) e = hipGraphInstantiate(&x, gr, nullptr, nullptr, 0);
+            if (gr) (void)hipGraphDestroy(gr);
+            if (r2) return r2;
+            if (e != hipSuccess) return hip_fail(e, "phase graph capture");
+            it = cache.emplace(key, x).first;
+        }
+        if (it != cache.end()) {
+            HIPCHK(hipGraphLaunch(it->second, stream));
+            graph_launches++;
+            HIPCHK(hipEventRecord(done[s], stream));
+            return CCFV_OK;
+        }
+    }
+    int rc = issue(ph, d);
+    if (rc) return rc;
+    eager_launches++;
+    HIPCHK(hipEventRecord(done[s], stream));
+    return CCFV_OK;
+}
+
+int PhaseRunner::issue(const Phase &ph, const char *d)
+{
     for (const Launch &L : ph.launches) {
         hipError_t e = hipSuccess;
         const int *pf = (const int *)(d + L.prefix);
@@ -226,7 +276,6 @@ int PhaseRunner::run(Phase &ph)
         }
         if (e != hipSuccess) return hip_fail(e, "ragged launch");
     }
-    HIPCHK(hipEventRecord(done[s], stream));
     return CCFV_OK;
 }
 

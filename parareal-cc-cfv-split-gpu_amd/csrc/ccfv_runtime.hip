@@ -1010,92 +1010,9 @@ int ccfv_autotune(ccfv_ctx *c, int32_t batch_fields, double h, int32_t out[8])
     if (batch_fields < 1 || batch_fields > 4096)
         return fail(CCFV_EINVAL, "batch_fields must be in [1, 4096]");
     if (!(h > 0.0)) return fail(CCFV_EINVAL, "h must be > 0");
-    const SolveTable *tx, *ty;
-    int rc = get_table(c, DIR_X, h, &tx);
-    if (rc) return rc;
-    rc = get_table(c, DIR_Y, h, &ty);
-    if (rc) return rc;
-    const bool post = post_flag(c);
-    const size_t N = c->ncell;
-    double *scratch = nullptr;
-    HIPCHK(hipMalloc(&scratch, sizeof(double) * N * (size_t)batch_fields));
-    for (int f = 0; f < batch_fields; f++) {
-        hipError_t e = launch_copy(c->state, scratch + f * N, N, c->stream);
-        if (e != hipSuccess) {
-            (void)hipFree(scratch);
-            return hip_fail(e, "autotune copy");
-        }
-    }
-    static const int ws[] = {1, 2, 4, 8, 16, 32};
-    static const int nws[] = {1, 2, 4};
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    (void)hipEventCreate(&e0);
-    (void)hipEventCreate(&e1);
-    int status = CCFV_OK;
-    for (int d = 0; d < 2 && !status; d++) {
-        const Dir dir = (Dir)d;
-        const int n = dir == DIR_X ? c->P.nx : c->P.ny;
-        const int nl = dir == DIR_X ? c->P.ny : c->P.nx;
-        for (int b = 0; b < 2 && !status; b++) {
-            const int nf = b ? batch_fields : 1;
-            const int reps = b ? 3 : 20;
-            // x: the launch the solves issue, the last x sub-step of a step
-            // fused with the first of the next (one sub-step when fusion is off)
-            const bool fused = dir == DIR_X && c->fuse;
-            SweepArgs a = make_args(c, dir, dir == DIR_X ? (fused ? 3 : 1) : 2,
-                                    dir == DIR_X ? *tx : *ty, h, 0.0, 0.0, 0, 0.0);
-            a.in = scratch;
-            a.out = scratch;
-            a.fstride = (long long)N;
-            if (fused) {
-                a.out2 = scratch;
-                a.field0_2 = 0;
-                a.t0_2 = 0.0;
-                a.jdt2 = h;
-            }
-            auto sweep = [&](Tile t) {
-                return fused ? run_sweep_fused(c, post, EPI_NONE, a, nf, t)
-                             : run_sweep(c, dir, false, EPI_NONE, a, nf, t);
-            };
-            float best = 1e30f;
-            Tile bt = c->tile[d][b];
-            for (int w : ws)
-                for (int nw : nws) {
-                    Tile t{w, nw};
-                    if (!tile_ok(t, n, nl, dir, post && dir == DIR_X,
-                                 c->P.src_kind != CCFV_SRC_NONE, c->P.bc))
-                        continue;
-                    float tbest = 1e30f;
-                    for (int trial = 0; trial < 2; trial++) {
-                        hipError_t e = sweep(t);
-                        if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
-                        for (int r = 0; r < reps && e == hipSuccess; r++) e = sweep(t);
-                        if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
-                        if (e == hipSuccess) e = hipEventSynchronize(e1);
-                        float ms = 0.f;
-                        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-                        if (e != hipSuccess) {
-                            status = hip_fail(e, "autotune sweep");
-                            break;
-                        }
-                        tbest = std::min(tbest, ms / reps);
-                    }
-                    if (status) break;
-                    if (tbest < best) {
-                        best = tbest;
-                        bt = t;
-                    }
-                }
-            c->tile[d][b] = bt;
-        }
-    }
-    clear_graphs(c);
-    c->tiles_version++;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipStreamSynchronize(c->stream);
-    (void)hipFree(scratch);
-    if (status) return status;
+    int rc;
+    for (int d = 0; d < 2; d++)
+        if ((rc = autotune_dir(c, (Dir)d, batch_fields, h))) return rc;
     if (out) ccfv_get_tiles(c, out);
     return check_flag(c);
 }
@@ -1385,6 +1302,94 @@ int parareal_sequential(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R
     }
     kdone = k;
     return CCFV_OK;
+}
+
+int autotune_dir(ccfv_ctx *c, Dir dir, int batch_fields, double h)
+{
+    const SolveTable *tb;
+    int rc = get_table(c, dir, h, &tb);
+    if (rc) return rc;
+    const bool post = post_flag(c);
+    const size_t N = c->ncell;
+    double *scratch = nullptr;
+    HIPCHK(hipMalloc(&scratch, sizeof(double) * N * (size_t)batch_fields));
+    for (int f = 0; f < batch_fields; f++) {
+        hipError_t e = launch_copy(c->state, scratch + f * N, N, c->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(scratch);
+            return hip_fail(e, "autotune copy");
+        }
+    }
+    static const int ws[] = {1, 2, 4, 8, 16, 32};
+    static const int nws[] = {1, 2, 4};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    hipStream_t keep = c->cur;
+    c->cur = c->stream;
+    int status = CCFV_OK;
+    const int n = dir == DIR_X ? c->P.nx : c->P.ny;
+    const int nl = dir == DIR_X ? c->P.ny : c->P.nx;
+    for (int b = 0; b < 2 && !status; b++) {
+        const int nf = b ? batch_fields : 1;
+        const int reps = b ? 3 : 20;
+        // x: the launch the solves issue, the last x sub-step of a step
+        // fused with the first of the next (one sub-step when fusion is off)
+        const bool fused = dir == DIR_X && c->fuse;
+        SweepArgs a = make_args(c, dir, dir == DIR_X ? (fused ? 3 : 1) : 2, *tb, h, 0.0, 0.0, 0,
+                                0.0);
+        a.in = scratch;
+        a.out = scratch;
+        a.fstride = (long long)N;
+        if (fused) {
+            a.out2 = scratch;
+            a.field0_2 = 0;
+            a.t0_2 = 0.0;
+            a.jdt2 = h;
+        }
+        auto sweep = [&](Tile t) {
+            return fused ? run_sweep_fused(c, post, EPI_NONE, a, nf, t)
+                         : run_sweep(c, dir, false, EPI_NONE, a, nf, t);
+        };
+        float best = 1e30f;
+        Tile bt = c->tile[dir][b];
+        for (int w : ws)
+            for (int nw : nws) {
+                Tile t{w, nw};
+                if (!tile_ok(t, n, nl, dir, post && dir == DIR_X, c->P.src_kind != CCFV_SRC_NONE,
+                             c->P.bc))
+                    continue;
+                float tbest = 1e30f;
+                for (int trial = 0; trial < 2; trial++) {
+                    hipError_t e = sweep(t);
+                    if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+                    for (int r = 0; r < reps && e == hipSuccess; r++) e = sweep(t);
+                    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+                    if (e == hipSuccess) e = hipEventSynchronize(e1);
+                    float ms = 0.f;
+                    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                    if (e != hipSuccess) {
+                        status = hip_fail(e, "autotune sweep");
+                        break;
+                    }
+                    tbest = std::min(tbest, ms / reps);
+                }
+                if (status) break;
+                if (tbest < best) {
+                    best = tbest;
+                    bt = t;
+                }
+            }
+        c->tile[dir][b] = bt;
+    }
+    c->cur = keep;
+    clear_graphs(c);
+    c->tiles_version++;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(scratch);
+    return status;
 }
 
 hipEvent_t sync_event(ccfv_ctx *c, size_t i)

@@ -168,3 +168,34 @@ def test_batch_small_plans(gpu, kmax, ns):
         assert reps[i].iterations == rr.iterations == min(kmax, ns)
         assert reps[i].converged == rr.converged
         assert reps[i].coarse_steps_run == ns + sum(ns - k - 1 for k in range(min(kmax, ns)))
+
+
+@pytest.mark.parametrize("tune", [False, True])
+def test_batch_graphs_and_tuned_shapes_match_solo(gpu, tune):
+    """Phases replayed as hipGraphs (the same launch lists recur across
+    solves and across the plain rounds), with and without shapes tuned per
+    line class: every result bitwise as solved alone, every replay as the
+    eager run."""
+    m = mix()[:5] + [mix()[7]]
+    nt = [64, 48, 36, 32, 40, 32]
+    ns = [8, 6, 4, 8, 5, 8]
+    t1 = [0.5, 0.25, 0.6, 0.25, 0.5, 0.5]
+    with gpu.Batch([p for p, _ in m]) as b:
+        if tune:
+            b.autotune(0.0, t1, nt, batch_fields=8)
+        b.set_option(T.OPT_GRAPHS, 1)
+        for rep_i in range(3):
+            for i, (_, ic) in enumerate(m):
+                b[i].fill_exact(ic, 0.0)
+            b.plain(0.0, t1, nt)
+            plain = [b[i].get_state() for i in range(len(m))]
+            for i, (_, ic) in enumerate(m):
+                b[i].fill_exact(ic, 0.0)
+            reps = b.parareal(0.0, t1, nt, ns, 3, 0.0)
+            para = [b[i].get_state() for i in range(len(m))]
+            for i, (p, ic) in enumerate(m):
+                assert np.array_equal(plain[i], solo_plain(gpu, p, ic, 0.0, t1[i], nt[i])), \
+                    (rep_i, i)
+                ref, rr = solo_parareal(gpu, p, ic, 0.0, t1[i], nt[i], ns[i], 3, 0.0)
+                assert np.array_equal(para[i], ref), (rep_i, i)
+                np.testing.assert_allclose(reps[i].increments, rr.increments, rtol=1e-12)
