@@ -59,7 +59,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # (profiles/rotation512_fine_sweep_pmc.md, profiles/rotation512_plain_pmc.md).
 # A calibrated constant, not a counter reading of this run.
 TRAFFIC_RATIO = {"x_batch": 1.002, "y_batch": 1.002, "y_single": 1.064,
-                 "x_epilog": 1.029}
+                 "x_single": 1.059, "x_epilog": 1.029}
 TRAFFIC_SOURCE = ("calibrated constant: rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1 per "
                   "dispatch over the algorithmic bytes (profiles/*_pmc.md); not read "
                   "in this run")
@@ -262,10 +262,11 @@ def rotation(a):
     log("rotation: parareal done")
 
     # one-field sweep classes of the plain solve, in isolation: 2048
-    # back-to-back launches (replayed from a hipGraph unless --no-graphs, as
-    # the plain-graph method issues them), beside the launch-rate floor of
-    # the same grid: an empty kernel, and a copy of the field with the
-    # sweep's memory pattern and no arithmetic
+    # back-to-back launches issued one by one (the duration rocprofv3 reports
+    # per dispatch) and replayed from a hipGraph (as the plain-graph method
+    # issues them), beside the launch-rate floor of the same grid: an empty
+    # kernel, and a copy of the field with the sweep's memory pattern and no
+    # arithmetic
     g = not a.no_graphs
     n_x, n_y = a.nt + 1, a.nt  # x launches: first x1, nt-1 fused, last x3
     bytes_one = 16.0 * cells
@@ -273,15 +274,16 @@ def rotation(a):
     for name, d, fused, kname in (("x_single", 0, True,
                                    "fast_x_kernel<8,0,true,0> (one field, two x sub-steps)"),
                                   ("y_single", 1, False, "fast_y_kernel<8,0> (one field)")):
-        us = ctx.time_sweep(d, fused, 1, dt, 2048, graph=g)
-        plain_classes[name] = {
-            "kernel": kname, "launches_per_solve": n_x if d == 0 else n_y,
-            "mean_us": us, "GBps": bytes_one / (us * 1e-6) / 1e9,
-            "empty_kernel_floor_us": ctx.dispatch_floor(d, 0, 2048, graph=g),
-            "copy_floor_us": ctx.dispatch_floor(d, 1, 2048, graph=g)}
-    floor = {"graph": g,
-             "x_empty_eager_us": ctx.dispatch_floor(0, 0, 2048),
-             "y_copy_eager_us": ctx.dispatch_floor(1, 1, 2048)}
+        us = ctx.time_sweep(d, fused, 1, dt, 2048)
+        c = {"kernel": kname, "launches_per_solve": n_x if d == 0 else n_y,
+             "mean_us": us, "GBps": bytes_one / (us * 1e-6) / 1e9,
+             "empty_kernel_floor_us": ctx.dispatch_floor(d, 0, 2048),
+             "copy_floor_us": ctx.dispatch_floor(d, 1, 2048)}
+        if g:
+            c["mean_us_graph"] = ctx.time_sweep(d, fused, 1, dt, 2048, graph=True)
+            c["empty_kernel_floor_us_graph"] = ctx.dispatch_floor(d, 0, 2048, graph=True)
+            c["copy_floor_us_graph"] = ctx.dispatch_floor(d, 1, 2048, graph=True)
+        plain_classes[name] = c
     log("rotation: isolated classes done")
 
     # Parareal classes, per launch, one instrumented solve on one stream
@@ -318,8 +320,7 @@ def rotation(a):
                 "bytes_per_launch": bytes_one,
                 "selection": "kernel class with the most time in the plain solve, timed in "
                              "isolation (ccfv_time_sweep: 2048 back-to-back launches "
-                             "between two hipEvents on the context's stream"
-                             + (", replayed from a hipGraph)" if g else ")"),
+                             "between two hipEvents on the context's stream)",
                 "copy_floor_us": c["copy_floor_us"],
                 "empty_kernel_floor_us": c["empty_kernel_floor_us"],
                 "frac_of_copy_floor": c["copy_floor_us"] / c["mean_us"],
@@ -360,7 +361,6 @@ def rotation(a):
         "roofline": roof,
         "roofline_parareal_fine_batch": fb,
         "kernel_classes": {"plain": plain_classes, "parareal": para_classes},
-        "dispatch_floor": floor,
     }
 
 

@@ -460,7 +460,7 @@ __device__ void rhs_general_reg(const SweepArgs &a, const Times &tt, const doubl
 #pragma unroll
     for (int c = 0; c < NP; c++) {
         double part[P];
-#pragma unroll 1
+#pragma unroll 2
         for (int k = 0; k < P; k++) {
             const int e = base + c * P + k;
             Face fr = eval_face<DIR, BC, false>(a, tt, cs, g, e + 1, lg, inv12d);
