@@ -559,6 +559,30 @@ void clear_graphs(ccfv_ctx *c)
     c->gcache.clear();
 }
 
+// A context keeps the tables of at most MAX_TABLES (direction, step size)
+// pairs.  Past that, the next one-context call drops them all before it
+// starts — with its streams idle and no capture in progress — together with
+// the captured graphs whose launches point at them; the call then builds
+// what it needs again.
+constexpr size_t MAX_TABLES = 32;
+
+int trim_tables(ccfv_ctx *c)
+{
+    if (c->tables.size() <= MAX_TABLES) return CCFV_OK;
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    HIPCHK(hipStreamIsCapturing(c->stream, &st));
+    if (st != hipStreamCaptureStatusNone) return CCFV_OK;
+    for (hipStream_t s : {c->stream, c->stream2, c->stream3})
+        if (s) HIPCHK(hipStreamSynchronize(s));
+    clear_graphs(c);
+    for (auto &kv : c->tables) {
+        HIPCHK(hipFree(kv.second.dev));
+        HIPCHK(hipFree(kv.second.line_dev));
+    }
+    c->tables.clear();
+    return CCFV_OK;
+}
+
 void free_para(ccfv_ctx *c)
 {
     clear_graphs(c);
@@ -855,7 +879,9 @@ int ccfv_step(ccfv_ctx *c, double tn, double h)
     if (!c) return fail(CCFV_EINVAL, "NULL context");
     if (!(h > 0.0) || !std::isfinite(h) || !std::isfinite(tn))
         return fail(CCFV_EINVAL, "step size must be finite and > 0");
-    int rc = step_fields(c, c->state, c->state, 1, tn, 0.0, 0, 0.0, h);
+    int rc = trim_tables(c);
+    if (rc) return rc;
+    rc = step_fields(c, c->state, c->state, 1, tn, 0.0, 0, 0.0, h);
     if (rc) return rc;
     return check_flag(c);
 }
@@ -874,7 +900,9 @@ int ccfv_fine(ccfv_ctx *c, double tn, double DT, int32_t J)
     if (!(DT > 0.0) || !std::isfinite(DT))
         return fail(CCFV_EINVAL, "DT must be finite and > 0");
     double dt = DT / (double)J;
-    int rc = run_steps(c, J, [&](int j) {
+    int rc = trim_tables(c);
+    if (rc) return rc;
+    rc = run_steps(c, J, [&](int j) {
         return StepDesc{c->state, c->state, 1, tn, 0.0, 0, j * dt, dt};
     });
     if (rc) return rc;
@@ -888,8 +916,10 @@ int ccfv_advance(ccfv_ctx *c, double t_origin, double dt, int32_t n_begin, int32
     if (!(dt > 0.0) || !std::isfinite(dt) || !std::isfinite(t_origin))
         return fail(CCFV_EINVAL, "step size must be finite and > 0");
     ccfv::rt::TraceRange tr("ccfv_advance");
+    int rc = trim_tables(c);
+    if (rc) return rc;
     c->cur = c->stream;
-    int rc = run_steps(c, n_end - n_begin, [&](int i) {
+    rc = run_steps(c, n_end - n_begin, [&](int i) {
         const int n = n_begin + i;
         return StepDesc{c->state, c->state, 1, t_origin + n * dt, 0.0, 0, 0.0, dt};
     });
@@ -906,6 +936,7 @@ int ccfv_plain_solve(ccfv_ctx *c, double t0, double t1, int32_t nt)
         return fail(CCFV_EINVAL, "need finite t1 > t0");
     double dt = (t1 - t0) / nt;
     ccfv::rt::TraceRange tr("ccfv_plain_solve");
+    if (int r0 = trim_tables(c)) return r0;
     c->cur = c->stream;
     auto body = [&]() -> int {
         return run_steps(c, nt, [&](int n) {
@@ -1814,7 +1845,9 @@ int ccfv_parareal_solve(ccfv_ctx *c, double t0, double t1, int32_t nt_fine,
         return fail(CCFV_EINVAL, "kmax must be in [0, 64]");
     if (!(t1 > t0)) return fail(CCFV_EINVAL, "need t1 > t0");
     ccfv::rt::TraceRange tr("ccfv_parareal_solve");
-    int rc = ensure_para(c, ns);
+    int rc = trim_tables(c);
+    if (rc) return rc;
+    rc = ensure_para(c, ns);
     if (rc) return rc;
     const size_t N = c->ncell;
     ParaPlan pl;
