@@ -284,6 +284,11 @@ def rotation(a):
             c["empty_kernel_floor_us_graph"] = ctx.dispatch_floor(d, 0, 2048, graph=True)
             c["copy_floor_us_graph"] = ctx.dispatch_floor(d, 1, 2048, graph=True)
         plain_classes[name] = c
+    # the plain solve's pattern without its arithmetic: y and x copies
+    # alternating in place, each reading what the previous one wrote
+    step_floor = {"launches": 2048, "us_per_step": 2 * ctx.dispatch_floor(0, 2, 2048)}
+    if g:
+        step_floor["us_per_step_graph"] = 2 * ctx.dispatch_floor(0, 2, 2048, graph=True)
     log("rotation: isolated classes done")
 
     # Parareal classes, per launch, one instrumented solve on one stream
@@ -361,6 +366,11 @@ def rotation(a):
         "roofline": roof,
         "roofline_parareal_fine_batch": fb,
         "kernel_classes": {"plain": plain_classes, "parareal": para_classes},
+        "plain_step_floor": dict(step_floor, **{
+            "measured_us_per_step": {m: 1e3 * methods[m]["p50_ms"] / a.nt
+                                     for m in methods if m.startswith("plain")},
+            "what": "two dependent copy launches per step with the sweeps' memory patterns "
+                    "and block mapping, in place, no arithmetic (ccfv_dispatch_floor kind 2)"}),
     }
 
 

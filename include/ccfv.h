@@ -372,8 +372,14 @@ int ccfv_time_sweep(ccfv_ctx *ctx, int32_t dir, int32_t fused, int32_t nfields, 
  * launch shape (one wave per line):
  *   kind 0: an empty kernel with that grid;
  *   kind 1: a copy of the state field into a scratch field with the sweep's
- *           memory pattern (y: each lane moves its contiguous chunk of a row;
- *           x: each lane its strided chunk of a column) and no arithmetic.
+ *           memory pattern and block-to-line mapping (y: each lane moves its
+ *           contiguous chunk of a row; x: each lane its strided chunk of a
+ *           column, column tiles dealt to the XCDs in contiguous runs) and
+ *           no arithmetic;
+ *   kind 2: a plain solve's pattern without its arithmetic: launches
+ *           alternate the y and the x copy of kind 1, each in place on one
+ *           scratch copy of the state, so every launch reads what the
+ *           previous one wrote (dir is ignored).
  * graph = 1 captures the launches as one hipGraph and times its replay.
  * *us_per_launch: hipEvent time / launches.  The state is not changed. */
 int ccfv_dispatch_floor(ccfv_ctx *ctx, int32_t dir, int32_t kind, int32_t launches,

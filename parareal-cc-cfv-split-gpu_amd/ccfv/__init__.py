@@ -305,7 +305,9 @@ class Context:
     def dispatch_floor(self, direction: int, kind: int, launches: int,
                        graph: bool = False) -> float:
         """µs per launch of back-to-back empty (kind 0) or copy (kind 1)
-        launches with the one-field sweep's grid (ccfv_dispatch_floor)."""
+        launches with the one-field sweep's grid, or of y and x copies
+        alternating in place (kind 2, a plain solve's pattern without its
+        arithmetic) (ccfv_dispatch_floor)."""
         us = ctypes.c_double()
         _check(load().ccfv_dispatch_floor(self._h, direction, kind, launches, int(graph),
                                           ctypes.byref(us)), "ccfv_dispatch_floor")

@@ -286,10 +286,11 @@ __global__ void floor_rows_kernel(const double *in, double *out, int nl, int n, 
     }
 }
 
-// x pattern: line j is column j (stride ld); per lane ch cells of the column
+// x pattern: line j is column j (stride ld); per lane ch cells of the column;
+// column tiles dealt to the XCDs as the x sweeps deal them (xcd_tile)
 __global__ void floor_cols_kernel(const double *in, double *out, int nl, int n, int ld, int ch)
 {
-    const int line = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int line = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (line >= nl || lane * ch >= n) return;
     const size_t base = (size_t)(lane * ch) * ld + line;

@@ -1167,9 +1167,10 @@ int ccfv_dispatch_floor(ccfv_ctx *c, int32_t dir, int32_t kind, int32_t launches
                         int32_t graph, double *us)
 {
     if (!c || !us) return fail(CCFV_EINVAL, "NULL argument");
-    if ((dir != DIR_X && dir != DIR_Y) || (kind != 0 && kind != 1) || launches < 1 ||
+    if (kind == 2) dir = DIR_X;
+    if ((dir != DIR_X && dir != DIR_Y) || kind < 0 || kind > 2 || launches < 1 ||
         launches > 1 << 20)
-        return fail(CCFV_EINVAL, "dir 0/1, kind 0/1, 1 <= launches <= 2^20");
+        return fail(CCFV_EINVAL, "dir 0/1, kind 0/1/2, 1 <= launches <= 2^20");
     const Dir d = (Dir)dir;
     const int n = d == DIR_X ? c->P.nx : c->P.ny, nl = d == DIR_X ? c->P.ny : c->P.nx;
     const Tile t = c->tile[d][0];
@@ -1179,10 +1180,23 @@ int ccfv_dispatch_floor(ccfv_ctx *c, int32_t dir, int32_t kind, int32_t launches
     hipGraphExec_t gx = nullptr;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess && kind == 2)
+        e = hipMemcpyAsync(scratch, c->state, sizeof(double) * c->ncell,
+                           hipMemcpyDeviceToDevice, c->stream);
     auto issue = [&]() {
         hipError_t r = hipSuccess;
-        for (int i = 0; i < launches && r == hipSuccess; i++)
-            r = launch_floor(kind, d, c->state, scratch, n, nl, c->P.ny, t, c->stream);
+        for (int i = 0; i < launches && r == hipSuccess; i++) {
+            if (kind == 2) {
+                // y copy then x copy, in place on the scratch field
+                const Dir di = (i & 1) ? DIR_X : DIR_Y;
+                const int ni = di == DIR_X ? c->P.nx : c->P.ny;
+                const int li = di == DIR_X ? c->P.ny : c->P.nx;
+                r = launch_floor(1, di, scratch, scratch, ni, li, c->P.ny, c->tile[di][0],
+                                 c->stream);
+            } else {
+                r = launch_floor(kind, d, c->state, scratch, n, nl, c->P.ny, t, c->stream);
+            }
+        }
         return r;
     };
     if (e == hipSuccess && graph) {
