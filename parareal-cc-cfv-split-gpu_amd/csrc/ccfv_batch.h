@@ -16,6 +16,7 @@ namespace rt {
 
 // one problem's contribution to one sweep stage
 struct StageItem {
+    int lane = 0;  // the problem's phase lane (ccfv_ctx::lane)
     Dir dir;
     int bc;
     bool post;
@@ -29,6 +30,7 @@ struct StageItem {
 // phase's descriptor arena
 struct Launch {
     enum Kind { SWEEP, CORR, REDUCE, COPY } kind;
+    int lane = 0;  // the stream it goes to (PhaseRunner::lanes)
     Dir dir = DIR_X;
     Tile tile{0, 0};
     int ch = 1, bc = 0;
@@ -46,22 +48,40 @@ struct Phase {
     void clear();
     size_t append(const void *p, size_t bytes);
     void add_sweep_stage(std::vector<StageItem> &items);
-    void add_corrections(const std::vector<RaggedCorr> &c, const std::vector<int> &blocks);
-    void add_reductions(const std::vector<RaggedReduce> &r);
-    void add_copies(const std::vector<RaggedCopy> &c);
+    // lanes: the lane of each item's problem; one launch per lane
+    void add_corrections(const std::vector<RaggedCorr> &c, const std::vector<int> &blocks,
+                         const std::vector<int> &lanes);
+    void add_reductions(const std::vector<RaggedReduce> &r, const std::vector<int> &lanes);
+    void add_copies(const std::vector<RaggedCopy> &c, const std::vector<int> &lanes);
     bool empty() const { return launches.empty(); }
 };
 
-// Issues phases on one stream.  The descriptors go through two pinned host +
-// device arenas used alternately, so building phase p+1 on the host overlaps
-// the device running phase p.
+// Issues phases.  The descriptors go through two pinned host + device
+// arenas used alternately, so building phase p+1 on the host overlaps the
+// device running phase p.
+// Lanes: every problem belongs to one of LANES lanes (ccfv_ctx::lane, by
+// grid class) and every launch holds the items of one lane, so launches of
+// different lanes touch different problems.  Each lane issues on its own
+// stream (lane 0 on `stream`): after the descriptor upload the other lanes
+// wait for `stream`, and at the end of the phase `stream` waits for them, so
+// the grid classes of a batch run side by side instead of one after another
+// within a phase, and everything after the phase on `stream` sees all of it.
 // With graphs on, the launches of a phase (not its descriptor upload) are
 // one hipGraph: a launch list seen a second time — same kernels, shapes,
-// item counts and descriptor offsets in the same arena — is captured and
-// from then on replayed with one graph launch; the kernels read the new
-// descriptors from the arena, so only the launch list has to repeat.
+// lanes, item counts and descriptor offsets in the same arena — is captured
+// (with the lanes' fork and join) and from then on replayed with one graph
+// launch; the kernels read the new descriptors from the arena, so only the
+// launch list has to repeat.
 struct PhaseRunner {
+    static constexpr int LANES = 4;
     hipStream_t stream = nullptr;
+    hipStream_t lanes[LANES] = {};  // lanes[0] == stream
+    hipEvent_t fork_ev = nullptr, join_ev[LANES] = {};
+    // grid classes (nx, ny) and their lanes, and the cells given to each
+    // lane so far (lane_for: a new class goes to the lane with the fewest)
+    std::map<std::pair<int, int>, int> class_lane;
+    double lane_cells[LANES] = {};
+    int lane_for(int nx, int ny);
     char *hbuf[2] = {nullptr, nullptr};
     char *dbuf[2] = {nullptr, nullptr};
     size_t cap[2] = {0, 0};
@@ -150,7 +170,7 @@ void finish_iteration(ParaJob &j, const double *sums_host);
 }  // namespace ccfv
 
 struct ccfv_batch {
-    std::vector<ccfv_ctx *> ctx;
+    std::vector<ccfv_ctx *> ctx;  // lane of each set by PhaseRunner::lane_for
     ccfv::rt::PhaseRunner run;
     // per-slot increment sums of every problem
     double *sums_dev = nullptr, *sums_host = nullptr;

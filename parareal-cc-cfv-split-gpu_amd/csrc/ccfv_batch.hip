@@ -68,6 +68,15 @@ int ccfv_batch_create(const ccfv_problem *probs, int32_t n, ccfv_batch **out)
         b->ctx.push_back(c);
     }
     int rc = b->run.init();
+    if (rc == CCFV_OK) {
+        // lanes by grid class, the largest classes first
+        std::vector<int> order(n);
+        for (int i = 0; i < n; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+            return (double)probs[a].nx * probs[a].ny > (double)probs[c].nx * probs[c].ny;
+        });
+        for (int i : order) b->ctx[i]->lane = b->run.lane_for(probs[i].nx, probs[i].ny);
+    }
     hipError_t e = hipSuccess;
     if (rc == CCFV_OK && ((e = hipEventCreate(&b->ev0)) != hipSuccess ||
                           (e = hipEventCreate(&b->ev1)) != hipSuccess))
@@ -176,8 +185,12 @@ int ccfv_batch_parareal(ccfv_batch *b, const double *t0, const double *t1,
     Phase ph;
     {
         std::vector<RaggedCopy> cp;
-        for (ParaJob &j : jobs) cp.push_back(RaggedCopy{j.c->state, j.c->U, j.c->ncell});
-        ph.add_copies(cp);
+        std::vector<int> lanes;
+        for (ParaJob &j : jobs) {
+            cp.push_back(RaggedCopy{j.c->state, j.c->U, j.c->ncell});
+            lanes.push_back(j.c->lane);
+        }
+        ph.add_copies(cp, lanes);
     }
     std::vector<ParaJob *> live;
     for (ParaJob &j : jobs) live.push_back(&j);
@@ -201,8 +214,12 @@ int ccfv_batch_parareal(ccfv_batch *b, const double *t0, const double *t1,
     }
     {
         std::vector<RaggedCopy> cp;
-        for (ParaJob &j : jobs) cp.push_back(RaggedCopy{j.result(), j.c->state, j.c->ncell});
-        ph.add_copies(cp);
+        std::vector<int> lanes;
+        for (ParaJob &j : jobs) {
+            cp.push_back(RaggedCopy{j.result(), j.c->state, j.c->ncell});
+            lanes.push_back(j.c->lane);
+        }
+        ph.add_copies(cp, lanes);
         if ((rc = b->run.run(ph))) return rc;
     }
     HIPCHK(hipEventRecord(b->ev1, s));

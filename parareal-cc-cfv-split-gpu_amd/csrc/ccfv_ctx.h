@@ -74,6 +74,9 @@ struct ccfv_ctx {
     double preset_h = 0.0;
     bool no_alloc = false;
     int tiles_version = 0;
+    // batches and the serving engine: the phase lane (stream) of this
+    // problem's launches
+    int lane = 0;
     std::map<std::string, hipGraphExec_t> gcache;
     std::vector<hipEvent_t> sync_ev;  // cross-stream events (no timing)
     // kernel timing

@@ -53,19 +53,19 @@ size_t Phase::append(const void *p, size_t bytes)
 // sweep items of one stage, grouped into launches
 void Phase::add_sweep_stage(std::vector<StageItem> &items)
 {
-    // one launch shares the kernel instance (line chunk, boundary mode, path,
-    // step-3 variant, epilogue) and the block shape
+    // one launch shares the lane, the kernel instance (line chunk, boundary
+    // mode, path, step-3 variant, epilogue) and the block shape
     struct Key {
-        int ch, bc, fast, post, epi, fuse, w, nw;
+        int lane, ch, bc, fast, post, epi, fuse, w, nw;
         bool operator<(const Key &o) const
         {
-            return std::tie(ch, bc, fast, post, epi, fuse, w, nw) <
-                   std::tie(o.ch, o.bc, o.fast, o.post, o.epi, o.fuse, o.w, o.nw);
+            return std::tie(lane, ch, bc, fast, post, epi, fuse, w, nw) <
+                   std::tie(o.lane, o.ch, o.bc, o.fast, o.post, o.epi, o.fuse, o.w, o.nw);
         }
     };
     std::map<Key, std::vector<const StageItem *>> groups;
     for (const StageItem &s : items) {
-        Key k{chunk_for(s.item.a.n), s.bc, s.item.a.lineS ? 1 : 0, s.post ? 1 : 0,
+        Key k{s.lane, chunk_for(s.item.a.n), s.bc, s.item.a.lineS ? 1 : 0, s.post ? 1 : 0,
               (int)s.epi, s.fuse ? 1 : 0, s.tile.w, s.tile.nw};
         groups[k].push_back(&s);
     }
@@ -88,6 +88,7 @@ void Phase::add_sweep_stage(std::vector<StageItem> &items)
         }
         Launch L;
         L.kind = Launch::SWEEP;
+        L.lane = k.lane;
         L.dir = kv.second.front()->dir;
         L.tile = kv.second.front()->tile;
         L.ch = k.ch;
@@ -105,49 +106,73 @@ void Phase::add_sweep_stage(std::vector<StageItem> &items)
     }
 }
 
-void Phase::add_corrections(const std::vector<RaggedCorr> &c, const std::vector<int> &blocks)
+void Phase::add_corrections(const std::vector<RaggedCorr> &c, const std::vector<int> &blocks,
+                            const std::vector<int> &lanes)
 {
-    if (c.empty()) return;
-    std::vector<int> prefix(1, 0);
-    for (int b : blocks) prefix.push_back(prefix.back() + b);
-    Launch L;
-    L.kind = Launch::CORR;
-    L.nitems = (int)c.size();
-    L.nblocks = prefix.back();
-    L.items = append(c.data(), sizeof(RaggedCorr) * c.size());
-    L.prefix = append(prefix.data(), sizeof(int) * prefix.size());
-    launches.push_back(L);
-}
-
-void Phase::add_reductions(const std::vector<RaggedReduce> &r)
-{
-    if (r.empty()) return;
-    std::vector<int> prefix(1, 0);
-    for (const RaggedReduce &x : r) prefix.push_back(prefix.back() + x.count);
-    Launch L;
-    L.kind = Launch::REDUCE;
-    L.nitems = (int)r.size();
-    L.nblocks = prefix.back();
-    L.items = append(r.data(), sizeof(RaggedReduce) * r.size());
-    L.prefix = append(prefix.data(), sizeof(int) * prefix.size());
-    launches.push_back(L);
-}
-
-void Phase::add_copies(const std::vector<RaggedCopy> &c)
-{
-    if (c.empty()) return;
-    std::vector<int> prefix(1, 0);
-    for (const RaggedCopy &x : c) {
-        size_t b = (x.n + 255) / 256;
-        prefix.push_back(prefix.back() + (int)std::min<size_t>(std::max<size_t>(b, 1), 1024));
+    for (int lane = 0; lane < PhaseRunner::LANES; lane++) {
+        std::vector<RaggedCorr> it;
+        std::vector<int> prefix(1, 0);
+        for (size_t i = 0; i < c.size(); i++)
+            if (lanes[i] == lane) {
+                it.push_back(c[i]);
+                prefix.push_back(prefix.back() + blocks[i]);
+            }
+        if (it.empty()) continue;
+        Launch L;
+        L.kind = Launch::CORR;
+        L.lane = lane;
+        L.nitems = (int)it.size();
+        L.nblocks = prefix.back();
+        L.items = append(it.data(), sizeof(RaggedCorr) * it.size());
+        L.prefix = append(prefix.data(), sizeof(int) * prefix.size());
+        launches.push_back(L);
     }
-    Launch L;
-    L.kind = Launch::COPY;
-    L.nitems = (int)c.size();
-    L.nblocks = prefix.back();
-    L.items = append(c.data(), sizeof(RaggedCopy) * c.size());
-    L.prefix = append(prefix.data(), sizeof(int) * prefix.size());
-    launches.push_back(L);
+}
+
+void Phase::add_reductions(const std::vector<RaggedReduce> &r, const std::vector<int> &lanes)
+{
+    for (int lane = 0; lane < PhaseRunner::LANES; lane++) {
+        std::vector<RaggedReduce> it;
+        std::vector<int> prefix(1, 0);
+        for (size_t i = 0; i < r.size(); i++)
+            if (lanes[i] == lane) {
+                it.push_back(r[i]);
+                prefix.push_back(prefix.back() + r[i].count);
+            }
+        if (it.empty()) continue;
+        Launch L;
+        L.kind = Launch::REDUCE;
+        L.lane = lane;
+        L.nitems = (int)it.size();
+        L.nblocks = prefix.back();
+        L.items = append(it.data(), sizeof(RaggedReduce) * it.size());
+        L.prefix = append(prefix.data(), sizeof(int) * prefix.size());
+        launches.push_back(L);
+    }
+}
+
+void Phase::add_copies(const std::vector<RaggedCopy> &c, const std::vector<int> &lanes)
+{
+    for (int lane = 0; lane < PhaseRunner::LANES; lane++) {
+        std::vector<RaggedCopy> it;
+        std::vector<int> prefix(1, 0);
+        for (size_t i = 0; i < c.size(); i++)
+            if (lanes[i] == lane) {
+                it.push_back(c[i]);
+                const size_t b = (c[i].n + 255) / 256;
+                prefix.push_back(prefix.back() +
+                                 (int)std::min<size_t>(std::max<size_t>(b, 1), 1024));
+            }
+        if (it.empty()) continue;
+        Launch L;
+        L.kind = Launch::COPY;
+        L.lane = lane;
+        L.nitems = (int)it.size();
+        L.nblocks = prefix.back();
+        L.items = append(it.data(), sizeof(RaggedCopy) * it.size());
+        L.prefix = append(prefix.data(), sizeof(int) * prefix.size());
+        launches.push_back(L);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -156,9 +181,27 @@ void Phase::add_copies(const std::vector<RaggedCopy> &c)
 int PhaseRunner::init()
 {
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    lanes[0] = stream;
+    for (int l = 1; l < LANES; l++) HIPCHK(hipStreamCreateWithFlags(&lanes[l], hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+    for (int l = 1; l < LANES; l++)
+        HIPCHK(hipEventCreateWithFlags(&join_ev[l], hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
     return CCFV_OK;
+}
+
+int PhaseRunner::lane_for(int nx, int ny)
+{
+    const auto key = std::make_pair(nx, ny);
+    auto it = class_lane.find(key);
+    if (it != class_lane.end()) return it->second;
+    int best = 0;
+    for (int l = 1; l < LANES; l++)
+        if (lane_cells[l] < lane_cells[best]) best = l;
+    lane_cells[best] += (double)nx * ny;
+    class_lane.emplace(key, best);
+    return best;
 }
 
 void PhaseRunner::clear_cache()
@@ -170,7 +213,8 @@ void PhaseRunner::clear_cache()
 
 void PhaseRunner::release()
 {
-    if (stream) (void)hipStreamSynchronize(stream);
+    for (int l = 0; l < LANES; l++)
+        if (lanes[l]) (void)hipStreamSynchronize(lanes[l]);
     clear_cache();
     for (int s = 0; s < 2; s++) {
         if (hbuf[s]) (void)hipHostFree(hbuf[s]);
@@ -180,8 +224,16 @@ void PhaseRunner::release()
         done[s] = nullptr;
         cap[s] = 0;
     }
+    for (int l = 1; l < LANES; l++) {
+        if (lanes[l]) (void)hipStreamDestroy(lanes[l]);
+        if (join_ev[l]) (void)hipEventDestroy(join_ev[l]);
+        lanes[l] = nullptr;
+        join_ev[l] = nullptr;
+    }
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    fork_ev = nullptr;
     if (stream) (void)hipStreamDestroy(stream);
-    stream = nullptr;
+    stream = lanes[0] = nullptr;
 }
 
 int PhaseRunner::run(Phase &ph)
@@ -213,8 +265,8 @@ int PhaseRunner::run(Phase &ph)
         // the launch list and the arena it reads
         std::string key((const char *)&d, sizeof(d));
         for (const Launch &L : ph.launches) {
-            const long long f[] = {L.kind, L.dir, L.tile.w, L.tile.nw, L.ch, L.bc, L.fast,
-                                   L.post, L.fuse, L.epi, L.nitems, L.nblocks,
+            const long long f[] = {L.kind, L.lane, L.dir, L.tile.w, L.tile.nw, L.ch, L.bc,
+                                   L.fast, L.post, L.fuse, L.epi, L.nitems, L.nblocks,
                                    (long long)L.lds, (long long)L.items, (long long)L.prefix};
             key.append((const char *)f, sizeof(f));
         }
@@ -248,34 +300,47 @@ int PhaseRunner::run(Phase &ph)
 
 int PhaseRunner::issue(const Phase &ph, const char *d)
 {
+    // fork: the lanes in use start after what `stream` holds (the upload)
+    bool used[LANES] = {};
+    for (const Launch &L : ph.launches) used[L.lane] = true;
+    HIPCHK(hipEventRecord(fork_ev, stream));
+    for (int l = 1; l < LANES; l++)
+        if (used[l]) HIPCHK(hipStreamWaitEvent(lanes[l], fork_ev, 0));
     for (const Launch &L : ph.launches) {
         hipError_t e = hipSuccess;
+        const hipStream_t ls = lanes[L.lane];
         const int *pf = (const int *)(d + L.prefix);
         switch (L.kind) {
         case Launch::SWEEP: {
             const RaggedItem *it = (const RaggedItem *)(d + L.items);
             e = L.fast ? launch_ragged_fast(L.dir, L.epi, L.ch, L.fuse, it, pf, L.nitems,
-                                            L.nblocks, L.tile, L.lds, stream)
+                                            L.nblocks, L.tile, L.lds, ls)
                        : launch_ragged_generic(L.dir, L.bc, L.post, L.epi, L.ch, L.fuse, it,
                                                pf, L.nitems, L.nblocks, L.tile, L.lds,
-                                               stream);
+                                               ls);
             break;
         }
         case Launch::CORR:
             e = launch_ragged_correction((const RaggedCorr *)(d + L.items), pf, L.nitems,
-                                         L.nblocks, stream);
+                                         L.nblocks, ls);
             break;
         case Launch::REDUCE:
             e = launch_ragged_reduce((const RaggedReduce *)(d + L.items), pf, L.nitems,
-                                     L.nblocks, stream);
+                                     L.nblocks, ls);
             break;
         case Launch::COPY:
             e = launch_ragged_copy((const RaggedCopy *)(d + L.items), pf, L.nitems,
-                                   L.nblocks, stream);
+                                   L.nblocks, ls);
             break;
         }
         if (e != hipSuccess) return hip_fail(e, "ragged launch");
     }
+    // join: `stream` continues after every lane's launches
+    for (int l = 1; l < LANES; l++)
+        if (used[l]) {
+            HIPCHK(hipEventRecord(join_ev[l], lanes[l]));
+            HIPCHK(hipStreamWaitEvent(stream, join_ev[l], 0));
+        }
     return CCFV_OK;
 }
 
@@ -288,6 +353,7 @@ StageItem stage_item(ccfv_ctx *c, Dir dir, int substep, const SolveTable &T,
                      const StepDesc &d, int shape)
 {
     StageItem s;
+    s.lane = c->lane;
     s.dir = dir;
     s.bc = c->P.bc;
     s.post = dir == DIR_X && substep == 3 && post_flag(c);
@@ -514,7 +580,9 @@ int append_iteration(Phase &ph, std::vector<ParaJob *> &jobs, double *sums_dev,
                                   p->c->partial + 2 * (size_t)p->nb * (k + 1)});
         cblocks.push_back(p->nb);
     }
-    ph.add_corrections(corr, cblocks);
+    std::vector<int> lanes;
+    for (ParaJob *p : it) lanes.push_back(p->c->lane);
+    ph.add_corrections(corr, cblocks, lanes);
     // coarse chain from slice k+1 with the correction fused into its last
     // sweep
     runs.clear();
@@ -533,7 +601,7 @@ int append_iteration(Phase &ph, std::vector<ParaJob *> &jobs, double *sums_dev,
         p->R.fine_steps_run += (double)(p->ns - p->k) * p->J;
         p->R.coarse_steps_run += p->ns - p->k - 1;
     }
-    ph.add_reductions(red);
+    ph.add_reductions(red, lanes);
     *used_out = used;
     return CCFV_OK;
 }

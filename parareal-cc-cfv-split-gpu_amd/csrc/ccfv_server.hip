@@ -102,6 +102,7 @@ int place(ccfv_server *s, Request &q)
     c.device = s->device;
     c.stream = c.stream2 = c.cur = s->run.stream;
     c.ncell = (size_t)c.P.nx * c.P.ny;
+    c.lane = s->run.lane_for(c.P.nx, c.P.ny);
     shared_tiles(&c);
     const int ns = q.r.n_slices;
     const size_t f = sizeof(double) * c.ncell;
