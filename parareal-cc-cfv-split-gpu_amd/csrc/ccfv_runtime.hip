@@ -1041,7 +1041,8 @@ int ccfv_autotune(ccfv_ctx *c, int32_t batch_fields, double h, int32_t out[8])
     if (batch_fields < 1 || batch_fields > 4096)
         return fail(CCFV_EINVAL, "batch_fields must be in [1, 4096]");
     if (!(h > 0.0)) return fail(CCFV_EINVAL, "h must be > 0");
-    int rc;
+    int rc = trim_tables(c);
+    if (rc) return rc;
     for (int d = 0; d < 2; d++)
         if ((rc = autotune_dir(c, (Dir)d, batch_fields, h))) return rc;
     if (out) ccfv_get_tiles(c, out);
@@ -1086,8 +1087,8 @@ int ccfv_time_sweep(ccfv_ctx *c, int32_t dir, int32_t fused, int32_t nf, double 
                                  "1 <= launches <= 2^20, h > 0, layout 0/1");
     const Dir d = (Dir)dir;
     const SolveTable *tb;
-    int rc = get_table(c, d, h, &tb);
-    if (rc) return rc;
+    int rc = trim_tables(c);
+    if (rc || (rc = get_table(c, d, h, &tb))) return rc;
     const int n = d == DIR_X ? c->P.nx : c->P.ny, nl = d == DIR_X ? c->P.ny : c->P.nx;
     const Tile t = (tile_w || tile_nw) ? Tile{tile_w, tile_nw} : c->tile[d][nf > 1 ? 1 : 0];
     if (!tile_ok(t, n, nl, d, post_flag(c) && d == DIR_X, c->P.src_kind != CCFV_SRC_NONE,
