@@ -1,6 +1,10 @@
 """The serving engine's admission bookkeeping (csrc/ccfv_sched.h: first-fit
 arena allocator, arrival-order admission with a bounded bypass, request
-footprint), unit-tested in C++ on the CPU."""
+footprint), unit-tested in C++ on the CPU: fixed cases, then seeded random
+operation sequences against a unit-by-unit model of the arena and against
+the admission contract (each check was seen to fail on a planted mistake
+in a copy of the header: an off-by-one bypass bound, no merge with the
+left neighbour, first fit skipping exact fits)."""
 import os
 import shutil
 import subprocess
