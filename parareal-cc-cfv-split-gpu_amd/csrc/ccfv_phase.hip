@@ -306,6 +306,7 @@ int PhaseRunner::issue(const Phase &ph, const char *d)
     HIPCHK(hipEventRecord(fork_ev, stream));
     for (int l = 1; l < LANES; l++)
         if (used[l]) HIPCHK(hipStreamWaitEvent(lanes[l], fork_ev, 0));
+    int rc = CCFV_OK;
     for (const Launch &L : ph.launches) {
         hipError_t e = hipSuccess;
         const hipStream_t ls = lanes[L.lane];
@@ -333,15 +334,20 @@ int PhaseRunner::issue(const Phase &ph, const char *d)
                                    L.nblocks, ls);
             break;
         }
-        if (e != hipSuccess) return hip_fail(e, "ragged launch");
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "ragged launch");
+            break;
+        }
     }
-    // join: `stream` continues after every lane's launches
+    // join: `stream` continues after every lane's launches (also after a
+    // failed launch, so that no lane is left outside a capture)
     for (int l = 1; l < LANES; l++)
         if (used[l]) {
-            HIPCHK(hipEventRecord(join_ev[l], lanes[l]));
-            HIPCHK(hipStreamWaitEvent(stream, join_ev[l], 0));
+            hipError_t e = hipEventRecord(join_ev[l], lanes[l]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(stream, join_ev[l], 0);
+            if (e != hipSuccess && rc == CCFV_OK) rc = hip_fail(e, "phase lane join");
         }
-    return CCFV_OK;
+    return rc;
 }
 
 // ---------------------------------------------------------------------------
