@@ -480,14 +480,20 @@ def ragged(a, reps=None, full=True):
     }
 
 
-def serve_stream(a, rate, adaptive):
+def serve_stream(a, rate, adaptive, serial=False):
     """One Poisson stream of a.count requests at `rate` through a fresh
     engine with the serial finish; every result checked against its serial
-    fine solution."""
+    fine solution.  serial: every request is submitted as the plain serial
+    solve (n_slices = 0)."""
     import ccfv
     from ccfv import serving, workloads
     stream = workloads.serving_stream(a.count, rate)
-    arr = [(t, s.request(a.kmax, a.tol), None) for t, s in stream]
+    arr = []
+    for t, s in stream:
+        r = s.request(a.kmax, a.tol)
+        if serial:
+            r.n_slices = 0
+        arr.append((t, r, None))
     with ccfv.Server(max_active=256, max_slices=64, adaptive_above=adaptive,
                      serial_finish=1) as srv:
         # warm-up request (module load, first launches)
@@ -504,7 +510,7 @@ def serve_stream(a, rate, adaptive):
         b.plain([s.t0 for s in sol], [s.t1 for s in sol], [s.nt_fine for s in sol])
         errs = [rel_l2(r.state, b[r.index].get_state()) for r in res.records]
     summ = res.summary()
-    summ.update({"policy": "adaptive" if adaptive else "as requested",
+    summ.update({"policy": "serial" if serial else ("adaptive" if adaptive else "as requested"),
                  "serial_finish": True, "rate": rate, "offered_s": stream[-1][0],
                  "ran_serially": int(sum(r.report.n_slices == 0 for r in res.records)),
                  "finished_serially": int(sum(r.report.serial_finish for r in res.records)),
@@ -519,16 +525,16 @@ def serve_stream(a, rate, adaptive):
 def serve(a):
     """Poisson request streams (ragged draws plus rotating-flow cases)
     through the serving engine, one stream per arrival rate, with requests
-    run as submitted (Parareal, serial finish) and with the adaptive policy
+    run as submitted (Parareal, serial finish), with the adaptive policy
     (requests that arrive while the engine is busy run as the plain serial
-    solve)."""
+    solve), and all submitted as plain serial solves."""
     out = {"metric": "serving: solves/s and p50/p99/p999 latency under Poisson "
                      "arrivals", "unit": "solves/s", "n_gpus": 1, "dtype": "f64",
            "data": "synthetic (seeded)", "kmax": a.kmax, "tol": a.tol,
            "adaptive_above": a.adaptive_above, "rates": []}
-    for adaptive in (0, a.adaptive_above):
+    for adaptive, serial in ((0, False), (a.adaptive_above, False), (0, True)):
         for rate in [float(x) for x in a.rates.split(",")]:
-            summ = serve_stream(a, rate, adaptive)
+            summ = serve_stream(a, rate, adaptive, serial)
             out["rates"].append(summ)
             log(json.dumps(summ))
     best = max(out["rates"], key=lambda r: r["solves_per_s"])
@@ -694,7 +700,8 @@ def main():
         log(f"leg ragged done ({time.perf_counter() - t:.1f} s)")
         t = time.perf_counter()
         legs["serve"] = {"as requested": serve_stream(a, a.leg_rate, 0),
-                         "adaptive": serve_stream(a, a.leg_rate, a.adaptive_above)}
+                         "adaptive": serve_stream(a, a.leg_rate, a.adaptive_above),
+                         "serial": serve_stream(a, a.leg_rate, 0, serial=True)}
         log(f"leg serve done ({time.perf_counter() - t:.1f} s)")
         line["legs"] = legs
     cpu = None
