@@ -269,32 +269,56 @@ __global__ void floor_empty_kernel(int *never)
     if (never && threadIdx.x == 1u << 30) *never = 1;  // no store is ever made
 }
 
-// y pattern: line i is row i (contiguous); per lane ch cells, 16-byte moves
-// (n is a multiple of ch, and even: lanes below n / ch move whole chunks)
-__global__ void floor_rows_kernel(const double *in, double *out, int nl, int n, int ld, int ch)
+// y pattern: line i is row i (contiguous); per lane CH cells, 16-byte moves.
+// Every load of a lane is issued before its first store, as in the sweeps
+// (in and out may be the same field).
+template <int CH>
+__global__ void floor_rows_kernel(const double *in, double *out, int nl, int n, int ld)
 {
     const int line = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (line >= nl || lane * ch >= n) return;
-    const double *src = in + (size_t)line * ld + lane * ch;
-    double *dst = out + (size_t)line * ld + lane * ch;
-    if (ch >= 2) {
-        for (int k = 0; k < ch; k += 2)
-            *reinterpret_cast<double2 *>(dst + k) = *reinterpret_cast<const double2 *>(src + k);
+    if (line >= nl || lane * CH >= n) return;
+    const double *src = in + (size_t)line * ld + lane * CH;
+    double *dst = out + (size_t)line * ld + lane * CH;
+    if constexpr (CH >= 2) {
+        double v[CH];
+#pragma unroll
+        for (int k = 0; k < CH / 2; k++) {
+            const double2 x = reinterpret_cast<const double2 *>(src)[k];
+            v[2 * k] = x.x;
+            v[2 * k + 1] = x.y;
+        }
+#pragma unroll
+        for (int k = 0; k < CH / 2; k++)
+            reinterpret_cast<double2 *>(dst)[k] = make_double2(v[2 * k], v[2 * k + 1]);
     } else {
         dst[0] = src[0];
     }
 }
 
-// x pattern: line j is column j (stride ld); per lane ch cells of the column;
+// x pattern: line j is column j (stride ld); per lane CH cells of the column;
 // column tiles dealt to the XCDs as the x sweeps deal them (xcd_tile)
-__global__ void floor_cols_kernel(const double *in, double *out, int nl, int n, int ld, int ch)
+template <int CH>
+__global__ void floor_cols_kernel(const double *in, double *out, int nl, int n, int ld)
 {
     const int line = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (line >= nl || lane * ch >= n) return;
-    const size_t base = (size_t)(lane * ch) * ld + line;
-    for (int k = 0; k < ch; k++) out[base + (size_t)k * ld] = in[base + (size_t)k * ld];
+    if (line >= nl || lane * CH >= n) return;
+    const size_t base = (size_t)(lane * CH) * ld + line;
+    double v[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) v[k] = in[base + (size_t)k * ld];
+#pragma unroll
+    for (int k = 0; k < CH; k++) out[base + (size_t)k * ld] = v[k];
+}
+
+template <int CH>
+void launch_floor_ch(int dir, const double *in, double *out, int nl, int n, int ld,
+                     dim3 grid, dim3 block, hipStream_t s)
+{
+    if (dir == DIR_Y)
+        hipLaunchKernelGGL(floor_rows_kernel<CH>, grid, block, 0, s, in, out, nl, n, ld);
+    else hipLaunchKernelGGL(floor_cols_kernel<CH>, grid, block, 0, s, in, out, nl, n, ld);
 }
 
 hipError_t launch_floor(int kind, int dir, const double *in, double *out, int n, int nl,
@@ -304,11 +328,19 @@ hipError_t launch_floor(int kind, int dir, const double *in, double *out, int n,
     // sweeps' shape, W == NW lines per block)
     const int nw = t.nw;
     const dim3 grid((unsigned)((nl + nw - 1) / nw)), block(64 * nw);
-    const int ch = chunk_for(n);
-    if (kind == 0) hipLaunchKernelGGL(floor_empty_kernel, grid, block, 0, s, nullptr);
-    else if (dir == DIR_Y)
-        hipLaunchKernelGGL(floor_rows_kernel, grid, block, 0, s, in, out, nl, n, ld, ch);
-    else hipLaunchKernelGGL(floor_cols_kernel, grid, block, 0, s, in, out, nl, n, ld, ch);
+    if (kind == 0) {
+        hipLaunchKernelGGL(floor_empty_kernel, grid, block, 0, s, nullptr);
+        return hipGetLastError();
+    }
+    switch (chunk_for(n)) {
+    case 1: launch_floor_ch<1>(dir, in, out, nl, n, ld, grid, block, s); break;
+    case 2: launch_floor_ch<2>(dir, in, out, nl, n, ld, grid, block, s); break;
+    case 4: launch_floor_ch<4>(dir, in, out, nl, n, ld, grid, block, s); break;
+    case 8: launch_floor_ch<8>(dir, in, out, nl, n, ld, grid, block, s); break;
+    case 16: launch_floor_ch<16>(dir, in, out, nl, n, ld, grid, block, s); break;
+    case 32: launch_floor_ch<32>(dir, in, out, nl, n, ld, grid, block, s); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
