@@ -33,7 +33,7 @@ struct Launch {
     int lane = 0;  // the stream it goes to (PhaseRunner::lanes)
     Dir dir = DIR_X;
     Tile tile{0, 0};
-    int ch = 1, bc = 0;
+    int ch = 1, wpl = 1, bc = 0;  // wpl: waves per line (split lines)
     bool fast = false, post = false, fuse = false;
     Epi epi = EPI_NONE;
     int nitems = 0, nblocks = 0;

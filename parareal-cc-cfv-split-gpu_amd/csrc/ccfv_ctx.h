@@ -124,8 +124,10 @@ Tile pick_tile(Dir dir, int n, int nl, bool batch, bool post, bool src);
 // batch_fields fields at step size h (copies of the state in scratch) and
 // keeps the fastest in c->tile[d][0] and c->tile[d][1]
 int autotune_dir(ccfv_ctx *c, Dir d, int batch_fields, double h);
-// lane-scan doubling steps of a table (its mneg and pneg parts)
-void scan_steps(const double *tab, int n, int &sf, int &sb);
+// lane-scan doubling steps of a table (its mneg and pneg parts): sf / sb
+// for the lanes of chunk_for(n) cells, sfs / sbs within the waves of a split
+// line (8-cell lanes)
+void scan_steps(const double *tab, int n, SolveTable &T);
 double cell_width(const ccfv_problem &P, Dir dir);
 int get_table(ccfv_ctx *c, Dir dir, double h, const SolveTable **out);
 SweepArgs make_args(ccfv_ctx *c, Dir dir, int substep, const SolveTable &T,

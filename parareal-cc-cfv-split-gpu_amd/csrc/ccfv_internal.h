@@ -23,7 +23,9 @@ enum Epi { EPI_NONE = 0, EPI_INIT = 1, EPI_CORRECT = 2 };
 // every line of a sweep, so they are computed once on the host).
 struct SolveTable {
     // [mneg(n) | den(n) | pneg(n) | z(n) | mf(64) | mb(64)]: Thomas factors
-    // of the constant matrix and the per-lane chunk products of mneg / pneg
+    // of the constant matrix and the per-lane chunk products of mneg / pneg;
+    // for split lines (fast_wpl(n) > 1) then [mfs | mbs], the products over
+    // the 64 wpl lanes of 8 cells (tab_doubles(n) in all)
     double *dev = nullptr;
     // uniform-displacement path: per line the cell shift S (as double) and the
     // five filter weights; fast_ok when every line passed the edge guard
@@ -34,7 +36,18 @@ struct SolveTable {
     double coef = 0;        // explicit diffusion factor (theta*nu)*(h/d)
     double vl = 0, smden = 1;  // Sherman-Morrison (periodic)
     int sf = 6, sb = 6;        // lane-scan doubling steps (forward, backward)
+    int sfs = 6, sbs = 6;      // the same for the 8-cell lanes of split lines
 };
+
+// Uniform-offset sweeps of 1024-cell lines run as split lines: two waves of
+// one workgroup per line, 8 cells per lane (every other line length: one
+// wave per line).  Waves of a line per line length:
+__host__ __device__ constexpr int fast_wpl(int n) { return n == 1024 ? 2 : 1; }
+// doubles of a solve table of lines of n cells (SolveTable::dev)
+__host__ __device__ constexpr int tab_doubles(int n)
+{
+    return 4 * n + 128 + (fast_wpl(n) > 1 ? 128 * fast_wpl(n) : 0);
+}
 
 // Everything one sweep kernel needs, passed by value.
 struct SweepArgs {
@@ -46,6 +59,7 @@ struct SweepArgs {
     int ld;             // row length of the field (ny)
     int quirk;          // reference ghost selection in the derivative stencil
     int sf, sb;         // lane-scan doubling steps, forward / backward
+    int sfs, sbs;       // the same within each wave of a split line
     double lo, hi, d;   // along-line box and cell width
     double olo, od;     // other-direction box start and cell width
     double eps;         // remap clip epsilon (closed box)
@@ -141,7 +155,7 @@ size_t sweep_lds_bytes_fast(int n, Tile tile);
 // ragged launches over many problems of one line class (same chunk CH,
 // boundary mode and path); items and prefix are device arrays.  fuse: x
 // launches that also run the next step's first x sub-step (SweepArgs.out2)
-hipError_t launch_ragged_fast(Dir dir, Epi epi, int ch, bool fuse,
+hipError_t launch_ragged_fast(Dir dir, Epi epi, int ch, int wpl, bool fuse,
                               const RaggedItem *items, const int *prefix, int nitems,
                               int nblocks, Tile tile, size_t lds, hipStream_t s);
 hipError_t launch_ragged_generic(Dir dir, int bc, bool post, Epi epi, int ch, bool fuse,

@@ -83,9 +83,9 @@ Layout layout_of(const ccfv_problem &P, bool fast)
     Layout L;
     size_t o = 32;  // the flag's 256 bytes
     L.tab[0] = o;
-    o = up(o + 4 * (size_t)P.nx + 128);
+    o = up(o + (size_t)tab_doubles(P.nx));
     L.tab[1] = o;
-    o = up(o + 4 * (size_t)P.ny + 128);
+    o = up(o + (size_t)tab_doubles(P.ny));
     L.work = o;
     o = up(o + 2 * (size_t)std::max(P.nx, P.ny));
     if (fast) {
@@ -105,7 +105,7 @@ Layout layout_of(const ccfv_problem &P, bool fast)
 // the host's share of a table: its scalars and scan steps (the device
 // builds the arrays with the same function), and whether every line takes
 // the uniform-offset sweep; fixed buffers, no allocation per call
-thread_local double h_tab[4 * 2048 + 128], h_work[2 * 2048];
+thread_local double h_tab[tab_doubles(2048)], h_work[2 * 2048];
 
 bool host_table(const ccfv_problem &P, Dir dir, double h, SolveTable &T)
 {
@@ -120,7 +120,7 @@ bool host_table(const ccfv_problem &P, Dir dir, double h, SolveTable &T)
     T.vl = S.vl;
     T.diag0 = S.diag0;
     T.smden = S.smden;
-    scan_steps(h_tab, n, T.sf, T.sb);
+    scan_steps(h_tab, n, T);
     if (!uniform_problem(P) || getenv("CCFV_NO_FAST")) return false;
     for (int L = 0; L < nl; L++) {
         double s, w[6];

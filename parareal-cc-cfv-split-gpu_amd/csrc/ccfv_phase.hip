@@ -53,20 +53,22 @@ size_t Phase::append(const void *p, size_t bytes)
 // sweep items of one stage, grouped into launches
 void Phase::add_sweep_stage(std::vector<StageItem> &items)
 {
-    // one launch shares the lane, the kernel instance (line chunk, boundary
-    // mode, path, step-3 variant, epilogue) and the block shape
+    // one launch shares the lane, the kernel instance (line chunk, waves per
+    // line, boundary mode, path, step-3 variant, epilogue) and the block shape
     struct Key {
-        int lane, ch, bc, fast, post, epi, fuse, w, nw;
+        int lane, ch, wpl, bc, fast, post, epi, fuse, w, nw;
         bool operator<(const Key &o) const
         {
-            return std::tie(lane, ch, bc, fast, post, epi, fuse, w, nw) <
-                   std::tie(o.lane, o.ch, o.bc, o.fast, o.post, o.epi, o.fuse, o.w, o.nw);
+            return std::tie(lane, ch, wpl, bc, fast, post, epi, fuse, w, nw) <
+                   std::tie(o.lane, o.ch, o.wpl, o.bc, o.fast, o.post, o.epi, o.fuse, o.w,
+                            o.nw);
         }
     };
     std::map<Key, std::vector<const StageItem *>> groups;
     for (const StageItem &s : items) {
-        Key k{s.lane, chunk_for(s.item.a.n), s.bc, s.item.a.lineS ? 1 : 0, s.post ? 1 : 0,
-              (int)s.epi, s.fuse ? 1 : 0, s.tile.w, s.tile.nw};
+        const bool fast = s.item.a.lineS != nullptr;
+        Key k{s.lane, chunk_for(s.item.a.n), fast ? fast_wpl(s.item.a.n) : 1, s.bc,
+              fast ? 1 : 0, s.post ? 1 : 0, (int)s.epi, s.fuse ? 1 : 0, s.tile.w, s.tile.nw};
         groups[k].push_back(&s);
     }
     for (auto &kv : groups) {
@@ -92,6 +94,7 @@ void Phase::add_sweep_stage(std::vector<StageItem> &items)
         L.dir = kv.second.front()->dir;
         L.tile = kv.second.front()->tile;
         L.ch = k.ch;
+        L.wpl = k.wpl;
         L.bc = k.bc;
         L.fast = k.fast != 0;
         L.post = k.post != 0;
@@ -265,8 +268,8 @@ int PhaseRunner::run(Phase &ph)
         // the launch list and the arena it reads
         std::string key((const char *)&d, sizeof(d));
         for (const Launch &L : ph.launches) {
-            const long long f[] = {L.kind, L.lane, L.dir, L.tile.w, L.tile.nw, L.ch, L.bc,
-                                   L.fast, L.post, L.fuse, L.epi, L.nitems, L.nblocks,
+            const long long f[] = {L.kind, L.lane, L.dir, L.tile.w, L.tile.nw, L.ch, L.wpl,
+                                   L.bc, L.fast, L.post, L.fuse, L.epi, L.nitems, L.nblocks,
                                    (long long)L.lds, (long long)L.items, (long long)L.prefix};
             key.append((const char *)f, sizeof(f));
         }
@@ -314,8 +317,8 @@ int PhaseRunner::issue(const Phase &ph, const char *d)
         switch (L.kind) {
         case Launch::SWEEP: {
             const RaggedItem *it = (const RaggedItem *)(d + L.items);
-            e = L.fast ? launch_ragged_fast(L.dir, L.epi, L.ch, L.fuse, it, pf, L.nitems,
-                                            L.nblocks, L.tile, L.lds, ls)
+            e = L.fast ? launch_ragged_fast(L.dir, L.epi, L.ch, L.wpl, L.fuse, it, pf,
+                                            L.nitems, L.nblocks, L.tile, L.lds, ls)
                        : launch_ragged_generic(L.dir, L.bc, L.post, L.epi, L.ch, L.fuse, it,
                                                pf, L.nitems, L.nblocks, L.tile, L.lds,
                                                ls);

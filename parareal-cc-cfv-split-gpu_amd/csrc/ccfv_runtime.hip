@@ -98,10 +98,11 @@ double cell_width(const ccfv_problem &P, Dir dir)
 
 // lane-scan doubling steps of a table: after s steps a lane's carry
 // neglects the product of the factors over 2^s whole chunks; stop once that
-// is below 1e-18 (far under FP64 rounding of the carried value)
-void scan_steps(const double *tab, int n, int &sf, int &sb)
+// is below 1e-18 (far under FP64 rounding of the carried value).  ch cells
+// per lane, the scan over at most 64 lanes (one wave).
+static void scan_steps_ch(const double *tab, int n, int ch, int &sf, int &sb)
 {
-    const int ch = chunk_for(n), nl = n / ch;
+    const int nl = n / ch, lanes = std::min(nl, 64);
     double pf = 0.0, pb = 0.0;
     for (int l = 0; l < nl; l++) {
         double f = 1.0, b = 1.0;
@@ -112,9 +113,9 @@ void scan_steps(const double *tab, int n, int &sf, int &sb)
         pf = std::max(pf, f);
         pb = std::max(pb, b);
     }
-    auto steps = [nl](double p) {
+    auto steps = [lanes](double p) {
         int full = 0;
-        while ((1 << full) < nl) full++;
+        while ((1 << full) < lanes) full++;
         if (full < 1) full = 1;
         if (p <= 0.0) return 1;
         if (p >= 1.0) return full;
@@ -127,6 +128,13 @@ void scan_steps(const double *tab, int n, int &sf, int &sb)
     sb = steps(pb);
 }
 
+void scan_steps(const double *tab, int n, SolveTable &T)
+{
+    scan_steps_ch(tab, n, chunk_for(n), T.sf, T.sb);
+    if (fast_wpl(n) > 1) scan_steps_ch(tab, n, 8, T.sfs, T.sbs);
+    else T.sfs = T.sf, T.sbs = T.sb;
+}
+
 // the scalars of a table (thomas_fill) and, in tab / lines, its arrays on
 // the host; lines: 7 nl doubles (shift per line, then 6 weights per line),
 // filled and *fast set when every line takes the uniform-offset sweep
@@ -136,7 +144,7 @@ void table_host(const ccfv_problem &P, Dir dir, double h, SolveTable &T,
     const int n = dir == DIR_X ? P.nx : P.ny;
     const int nl = dir == DIR_X ? P.ny : P.nx;
     std::vector<double> work(2 * (size_t)n);
-    tab.assign(4 * (size_t)n + 128, 1.0);
+    tab.assign((size_t)tab_doubles(n), 1.0);
     const TabScalars S =
         thomas_fill(n, cell_width(P, dir), dir == DIR_X ? 0.25 : 0.5, P.nu, h,
                     P.bc == CCFV_BC_PERIODIC, chunk_for(n), tab.data(), work.data());
@@ -146,7 +154,7 @@ void table_host(const ccfv_problem &P, Dir dir, double h, SolveTable &T,
     T.vl = S.vl;
     T.diag0 = S.diag0;
     T.smden = S.smden;
-    scan_steps(tab.data(), n, T.sf, T.sb);
+    scan_steps(tab.data(), n, T);
     *fast = false;
     lines.clear();
     if (!uniform_problem(P) || getenv("CCFV_NO_FAST")) return;
@@ -222,6 +230,8 @@ SweepArgs make_args(ccfv_ctx *c, Dir dir, int substep, const SolveTable &T,
     a.quirk = (P.flags & CCFV_FLAG_DERIV_GHOST_QUIRK) ? 1 : 0;
     a.sf = T.sf;
     a.sb = T.sb;
+    a.sfs = T.sfs;
+    a.sbs = T.sbs;
     a.coef = T.coef;
     a.ca = T.ca;
     a.diag0 = T.diag0;

@@ -696,6 +696,406 @@ __device__ __forceinline__ void fast_y_body(const SweepArgs &a, int field, int t
     else store_tile_y(a.out + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
 }
 
+// ---------------------------------------------------------------------------
+// Split lines: a line of n = 64 CH WPL cells swept by WPL waves of one
+// workgroup, wave s of the line's group holding segment s (global lane
+// g = 64 s + lane, cells CH g .. CH g + CH - 1).  1024-cell lines take it
+// with CH = 8: one wave per such line held 16 cells per lane, its Thomas
+// factors in 96 registers (two waves per SIMD), and twice the serial chain
+// of a 512-cell line.  The segments meet through a few LDS slots after the
+// line (SplitSlots): the compact operator's neighbours at the segment ends,
+// the forward and backward chunk maps of the segment end lanes, and the
+// Sherman-Morrison end values; five workgroup barriers per sweep, which
+// every wave of the workgroup reaches (each group sweeps W / G lines).
+// The lane scans run within each wave for sfs / sbs doubling steps (the
+// truncation of the one-wave sweep; lanes below the step keep their exact
+// prefix), and a segment's carry-in folds the end maps of the segments
+// before it exactly, so a line's solve is the one-wave solve up to the
+// products the truncation neglects (below 1e-18).
+// ---------------------------------------------------------------------------
+template <int WPL>
+struct SplitSlots {
+    // per segment: b of its first / last cell, the forward map (Q, M) of
+    // its last lane, the backward map of its first lane; then y_0, y_{n-1}
+    static constexpr int BF = 0, BL = WPL, QF = 2 * WPL, MF = 3 * WPL, QB = 4 * WPL,
+                         MB = 5 * WPL, Y0 = 6 * WPL, YN = 6 * WPL + 1, SIZE = 6 * WPL + 2;
+};
+
+// LDS doubles per line: the cells (pidx layout), the ghost chunks, the slots
+template <int CH, int WPL>
+__host__ __device__ constexpr int split_pitch()
+{
+    return 64 * CH * WPL + 8 * ((CH + 11) >> 3) + ((64 * CH * WPL + 8 * ((CH + 11) >> 3)) >> 3) +
+           1 + SplitSlots<WPL>::SIZE;
+}
+
+// res: when given, the lane's CH results go there instead of back to cs
+template <int CH, int WPL>
+__device__ __forceinline__ void fast_line_split(const SweepArgs &a, double *cs,
+                                                const LineK &lk, int seg, int lane,
+                                                const LaneTab<CH, true> &tb, double mfl,
+                                                double mbl, double pf, double pl,
+                                                double (*res)[CH] = nullptr)
+{
+    static_assert(CH == 8, "split lines: lanes of one pidx chunk");
+    using X = SplitSlots<WPL>;
+    constexpr int n = 64 * CH * WPL, NC = n >> 3;
+    double *xc = cs + line_pitch(n + fast_ghost_cells(CH));
+    const int base = (seg * 64 + lane) * CH;
+
+    // ghost chunks after cell n-1 (segment 0 holds cells [0, FAST_GHOST));
+    // then every segment's cells are in place
+    if (seg == 0 && lane < fast_ghost_cells(CH)) cs[pidx(lane) + 9 * NC] = cs[pidx(lane)];
+    __syncthreads();
+
+    int s2 = (lk.S - 2) % n;
+    s2 = s2 < 0 ? s2 + n : s2;
+    int cb = (seg * 64 + lane) * (CH / 8) + (s2 >> 3);
+    cb = cb >= NC ? cb - NC : cb;
+    double b[CH];
+    {
+        double cw[CH + 5];
+        load_window<CH>(cs + 9 * cb, s2 & 7, cw);
+        filter6<CH>(lk, cw, b);
+    }
+
+    // compact operator, periodic (T2S4_Parareal.cu:616-654 with wrap-around);
+    // past this barrier no wave reads the line's cells again
+    if (lane == 0) xc[X::BF + seg] = b[0];
+    if (lane == 63) xc[X::BL + seg] = b[CH - 1];
+    __syncthreads();
+    double bprev = dpp_d<DPP_WAVE_SHR1>(b[CH - 1]);
+    double bnext = dpp_d<DPP_WAVE_SHL1>(b[0]);
+    if (lane == 0) bprev = xc[X::BL + (seg == 0 ? WPL - 1 : seg - 1)];
+    if (lane == 63) bnext = xc[X::BF + (seg == WPL - 1 ? 0 : seg + 1)];
+    double r[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        double bn = k + 1 < CH ? b[k + 1] : bnext;
+        double bp = k > 0 ? b[k - 1] : bprev;
+        r[k] = bn + 10.0 * b[k] + bp;  // the 1/12 is in den
+    }
+
+    // forward pass q_e = r_e den_e + mneg_e q_{e-1} (mneg_0 = 0)
+    double M = mfl, Q = 0.0;
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        double c = r[k] * tb.D(k, base + k);
+        r[k] = c;
+        Q = c + tb.M(k, base + k) * Q;
+    }
+    for (int st = 0, off = 1; st < a.sfs; st++, off <<= 1) {
+        double Mp, Qp;
+        if (off == 1) {
+            Mp = dpp_d<DPP_WAVE_SHR1>(M);
+            Qp = dpp_d<DPP_WAVE_SHR1>(Q);
+        } else {
+            Mp = __shfl_up(M, off);
+            Qp = __shfl_up(Q, off);
+        }
+        if (lane >= off) {
+            Q = M * Qp + Q;
+            M = M * Mp;
+        }
+    }
+    if (lane == 63) {
+        xc[X::QF + seg] = Q;
+        xc[X::MF + seg] = M;
+    }
+    __syncthreads();
+    double cin = 0.0;
+    for (int s = 0; s < seg; s++) cin = xc[X::QF + s] + xc[X::MF + s] * cin;
+    double carry;
+    {
+        const double Qs = dpp_d<DPP_WAVE_SHR1>(Q), Ms = dpp_d<DPP_WAVE_SHR1>(M);
+        carry = lane == 0 ? cin : Qs + Ms * cin;
+    }
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        carry = r[k] + tb.M(k, base + k) * carry;
+        r[k] = carry;
+    }
+
+    // back substitution x_e = q_e + pneg_e x_{e+1}
+    M = mbl;
+    Q = 0.0;
+#pragma unroll
+    for (int k = CH - 1; k >= 0; k--) {
+        const double pn = k == 0 ? pf : (k == CH - 1 ? pl : tb.M(k, base + k));
+        Q = r[k] + pn * Q;
+    }
+    for (int st = 0, off = 1; st < a.sbs; st++, off <<= 1) {
+        double Mn, Qn;
+        if (off == 1) {
+            Mn = dpp_d<DPP_WAVE_SHL1>(M);
+            Qn = dpp_d<DPP_WAVE_SHL1>(Q);
+        } else {
+            Mn = __shfl_down(M, off);
+            Qn = __shfl_down(Q, off);
+        }
+        if (lane + off < 64) {
+            Q = M * Qn + Q;
+            M = M * Mn;
+        }
+    }
+    if (lane == 0) {
+        xc[X::QB + seg] = Q;
+        xc[X::MB + seg] = M;
+    }
+    __syncthreads();
+    double cinb = 0.0;
+    for (int s = WPL - 1; s > seg; s--) cinb = xc[X::QB + s] + xc[X::MB + s] * cinb;
+    {
+        const double Qs = dpp_d<DPP_WAVE_SHL1>(Q), Ms = dpp_d<DPP_WAVE_SHL1>(M);
+        carry = lane == 63 ? cinb : Qs + Ms * cinb;
+    }
+#pragma unroll
+    for (int k = CH - 1; k >= 0; k--) {
+        const double pn = k == 0 ? pf : (k == CH - 1 ? pl : tb.M(k, base + k));
+        carry = r[k] + pn * carry;
+        r[k] = carry;
+    }
+
+    // Sherman-Morrison: x = y - ((y_0 + vl y_{n-1}) / (1 + v.z)) z
+    if (seg == 0 && lane == 0) xc[X::Y0] = r[0];
+    if (seg == WPL - 1 && lane == 63) xc[X::YN] = r[CH - 1];
+    __syncthreads();
+    const double fac = (xc[X::Y0] + a.vl * xc[X::YN]) * a.smrcp;
+    if (res) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) (*res)[k] = r[k] - fac * tb.Z(k, base + k);
+    } else {
+#pragma unroll
+        for (int k = 0; k < CH; k++) cs[pidx(base + k)] = r[k] - fac * tb.Z(k, base + k);
+    }
+}
+
+// a split-line lane's table values: Thomas factors of its cells, chunk
+// products, back-substitution factors of its end cells
+template <int CH, int WPL>
+struct SplitLane {
+    LaneTab<CH, true> tb;
+    double mfl, mbl, pf, pl;
+    __device__ __forceinline__ void load(const SweepArgs &a, int seg, int lane)
+    {
+        constexpr int n = 64 * CH * WPL, NL = 64 * WPL;
+        const int g = seg * 64 + lane;
+        tb.load(a, g * CH, true);
+        mfl = a.tab[4 * n + 128 + g];
+        mbl = a.tab[4 * n + 128 + NL + g];
+        pf = g == 0 ? a.tab[2 * n] : tb.M(0, g * CH);
+        pl = g == NL - 1 ? 0.0 : tb.M(CH - 1, g * CH + CH - 1);
+    }
+};
+
+// lines group, group + G, ... of a tile whose first line is L0 (every group
+// sweeps W / G lines, so every wave reaches the same barriers)
+template <int CH, int WPL>
+__device__ __forceinline__ void sweep_lines_split(const SweepArgs &a, double *lds, int LP,
+                                                  int L0, int W, int G, int group, int seg,
+                                                  int lane, const SplitLane<CH, WPL> &sl)
+{
+    LineK cur = line_k(a, L0 + group);
+    for (int l = group; l < W; l += G) {
+        LineK nxt = cur;
+        if (l + G < W) nxt = line_k(a, L0 + l + G);
+        fast_line_split<CH, WPL>(a, lds + l * LP, cur, seg, lane, sl.tb, sl.mfl, sl.mbl, sl.pf,
+                                 sl.pl);
+        cur = nxt;
+    }
+}
+
+// one-field x launch of split lines, one line per group (W == G): each lane
+// gathers its CH cells of the column, the line's segments meet in LDS, the
+// results are scattered back from registers
+template <int CH, int WPL, bool FUSE>
+__device__ __forceinline__ void fast_x_direct_split(const SweepArgs &a, int field, int tile,
+                                                   int W, double *lds)
+{
+    constexpr int LP = split_pitch<CH, WPL>();
+    const size_t foff = (size_t)field * (size_t)a.fstride;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int group = wave / WPL, seg = wave - group * WPL;
+    const int j = tile * W + group;
+    double *cs = lds + group * LP;
+    const int g = seg * 64 + lane;
+    const size_t row0 = (size_t)(g * CH) * a.ld;
+    const double *in = a.in + foff + j + row0;
+    double v[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) v[k] = in[(size_t)k * a.ld];
+    SplitLane<CH, WPL> sl;
+    sl.load(a, seg, lane);
+    const LineK lk = line_k(a, j);
+    double *own = cs + pidx(CH * g);
+#pragma unroll
+    for (int k = 0; k < CH; k++) own[k + (k >> 3)] = v[k];
+    double res[CH];
+    if (FUSE) fast_line_split<CH, WPL>(a, cs, lk, seg, lane, sl.tb, sl.mfl, sl.mbl, sl.pf, sl.pl);
+    fast_line_split<CH, WPL>(a, cs, lk, seg, lane, sl.tb, sl.mfl, sl.mbl, sl.pf, sl.pl, &res);
+    double *out = (FUSE ? a.out2 : a.out) + foff + j + row0;
+#pragma unroll
+    for (int k = 0; k < CH; k++) out[(size_t)k * a.ld] = res[k];
+}
+
+// one-field y launch of split lines (row-major, W == G): each lane moves
+// its CH contiguous cells between HBM and registers
+template <int CH, int WPL>
+__device__ __forceinline__ void fast_y_direct_split(const SweepArgs &a, int field, int tile,
+                                                   int W, double *lds)
+{
+    constexpr int LP = split_pitch<CH, WPL>();
+    const size_t foff = (size_t)field * (size_t)a.fstride;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int group = wave / WPL, seg = wave - group * WPL;
+    const int i = tile * W + group;
+    double *cs = lds + group * LP;
+    const int g = seg * 64 + lane;
+    const double2 *src =
+        reinterpret_cast<const double2 *>(a.in + foff + (size_t)i * a.ld + g * CH);
+    double2 v[CH / 2];
+#pragma unroll
+    for (int k = 0; k < CH / 2; k++) v[k] = src[k];
+    SplitLane<CH, WPL> sl;
+    sl.load(a, seg, lane);
+    const LineK lk = line_k(a, i);
+    double *own = cs + pidx(CH * g);
+#pragma unroll
+    for (int k = 0; k < CH / 2; k++) {
+        own[2 * k + (2 * k >> 3)] = v[k].x;
+        own[2 * k + 1 + ((2 * k + 1) >> 3)] = v[k].y;
+    }
+    double res[CH];
+    fast_line_split<CH, WPL>(a, cs, lk, seg, lane, sl.tb, sl.mfl, sl.mbl, sl.pf, sl.pl, &res);
+    double2 *dst = reinterpret_cast<double2 *>(a.out + foff + (size_t)i * a.ld + g * CH);
+#pragma unroll
+    for (int k = 0; k < CH / 2; k++) dst[k] = make_double2(res[2 * k], res[2 * k + 1]);
+}
+
+// the x launch of split lines: tiles of W lines, G groups of WPL waves (the
+// host's tile.nw is G); the arrangement of fast_x_body otherwise
+template <int CH, int WPL, int EPI, bool FUSE, int LAYOUT = 0>
+__device__ __forceinline__ void fast_x_body_split(const SweepArgs &a, int field, int tile,
+                                                 int W, int G, double *lds, bool direct)
+{
+    static_assert(LAYOUT == 0 || EPI == EPI_NONE, "epilogues write row-major fields");
+    constexpr bool pin = (LAYOUT & LAYOUT_IN) != 0, pout = (LAYOUT & LAYOUT_OUT) != 0;
+    if constexpr (EPI == EPI_NONE && LAYOUT == 0) {
+        if (direct && W == G) {
+            fast_x_direct_split<CH, WPL, FUSE>(a, field, tile, W, lds);
+            return;
+        }
+    }
+    constexpr int n = 64 * CH * WPL, LP = split_pitch<CH, WPL>();
+    const size_t foff = (size_t)field * (size_t)a.fstride;
+    const int j0 = tile * W;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int group = wave / WPL, seg = wave - group * WPL;
+    SplitLane<CH, WPL> sl;
+    sl.load(a, seg, lane);
+    EpiInputs pre;
+    if (EPI == EPI_CORRECT) epi_load(a, n, j0, W, threadIdx.x, pre);
+    if constexpr (pin) load_tile_x(panel_x_base(a.in + foff, n, j0), PANEL_W, n, j0, W, lds, LP);
+    else load_tile_x(a.in + foff, a.ld, n, j0, W, lds, LP);
+    __syncthreads();
+    sweep_lines_split<CH, WPL>(a, lds, LP, j0, W, G, group, seg, lane, sl);
+    __syncthreads();
+    if (!FUSE) {
+        if constexpr (EPI != EPI_NONE) epilogue_x<EPI>(a, n, j0, W, lds, LP, tile, &pre);
+        else if constexpr (pout)
+            store_tile_x(panel_x_base(a.out + foff, n, j0), PANEL_W, n, j0, W, lds, LP);
+        else store_tile_x(a.out + foff, a.ld, n, j0, W, lds, LP);
+        return;
+    }
+    if (EPI != EPI_NONE) {
+        epilogue_x<EPI, true>(a, n, j0, W, lds, LP, tile, &pre);
+        __syncthreads();
+    }
+    sweep_lines_split<CH, WPL>(a, lds, LP, j0, W, G, group, seg, lane, sl);
+    __syncthreads();
+    if constexpr (pout)
+        store_tile_x(panel_x_base(a.out2 + foff, n, j0), PANEL_W, n, j0, W, lds, LP);
+    else store_tile_x(a.out2 + foff, a.ld, n, j0, W, lds, LP);
+}
+
+template <int CH, int WPL, int LAYOUT = 0>
+__device__ __forceinline__ void fast_y_body_split(const SweepArgs &a, int field, int tile,
+                                                 int W, int G, double *lds, bool direct)
+{
+    constexpr bool pin = (LAYOUT & LAYOUT_IN) != 0, pout = (LAYOUT & LAYOUT_OUT) != 0;
+    if constexpr (LAYOUT == 0) {
+        if (direct && W == G) {
+            fast_y_direct_split<CH, WPL>(a, field, tile, W, lds);
+            return;
+        }
+    }
+    constexpr int n = 64 * CH * WPL, LP = split_pitch<CH, WPL>();
+    const size_t foff = (size_t)field * (size_t)a.fstride;
+    const int i0 = tile * W;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int group = wave / WPL, seg = wave - group * WPL;
+    SplitLane<CH, WPL> sl;
+    sl.load(a, seg, lane);
+    const size_t ps = (size_t)PANEL_W * (size_t)a.nl;
+    if constexpr (pin) load_tile_y_panel(a.in + foff, ps, n, i0, W, lds, LP);
+    else load_tile_y(a.in + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
+    __syncthreads();
+    sweep_lines_split<CH, WPL>(a, lds, LP, i0, W, G, group, seg, lane, sl);
+    __syncthreads();
+    if constexpr (pout) store_tile_y_panel(a.out + foff, ps, n, i0, W, lds, LP);
+    else store_tile_y(a.out + foff + (size_t)i0 * a.ld, a.ld, n, W, lds, LP);
+}
+
+template <int CH, int WPL, int EPI, bool FUSE, int LAYOUT = 0>
+__global__ __launch_bounds__(256 * WPL) void fast_x_split_kernel(SweepArgs a, int W, int G)
+{
+    extern __shared__ double lds[];
+    fast_x_body_split<CH, WPL, EPI, FUSE, LAYOUT>(a, blockIdx.y,
+                                                  xcd_tile(blockIdx.x, gridDim.x), W, G, lds,
+                                                  gridDim.y == 1);
+}
+
+template <int CH, int WPL, int LAYOUT = 0>
+__global__ __launch_bounds__(256 * WPL) void fast_y_split_kernel(SweepArgs a, int W, int G)
+{
+    extern __shared__ double lds[];
+    fast_y_body_split<CH, WPL, LAYOUT>(a, blockIdx.y, blockIdx.x, W, G, lds, gridDim.y == 1);
+}
+
+template <int CH, int WPL, int EPI, bool FUSE>
+__global__ __launch_bounds__(256 * WPL) void ragged_fast_x_split(const RaggedItem *items,
+                                                                 const int *prefix, int nitems,
+                                                                 int W, int G)
+{
+    extern __shared__ double lds[];
+    const int b = blockIdx.x;
+    const int i = __builtin_amdgcn_readfirstlane(ragged_item(prefix, nitems, b));
+    const int local = ragged_xcd_tile(b - prefix[i], prefix[i], prefix[i + 1] - prefix[i]);
+    const int nt = items[i].ntiles;
+    const int field = local / nt;
+    const SweepArgs a = items[i].a;
+    fast_x_body_split<CH, WPL, EPI, FUSE>(a, field, local - field * nt, W, G, lds, false);
+}
+
+template <int CH, int WPL>
+__global__ __launch_bounds__(256 * WPL) void ragged_fast_y_split(const RaggedItem *items,
+                                                                 const int *prefix, int nitems,
+                                                                 int W, int G)
+{
+    extern __shared__ double lds[];
+    const int b = blockIdx.x;
+    const int i = __builtin_amdgcn_readfirstlane(ragged_item(prefix, nitems, b));
+    const int local = b - prefix[i];
+    const int nt = items[i].ntiles;
+    const int field = local / nt;
+    const SweepArgs a = items[i].a;
+    fast_y_body_split<CH, WPL>(a, field, local - field * nt, W, G, lds, false);
+}
+
 template <int CH, int EPI, bool FUSE, int LAYOUT = 0>
 __global__ __launch_bounds__(256) void fast_x_kernel(SweepArgs a, int W, int NW)
 {
@@ -744,6 +1144,7 @@ __global__ __launch_bounds__(256) void ragged_fast_y(const RaggedItem *items,
 
 size_t sweep_lds_bytes_fast(int n, Tile tile)
 {
+    if (fast_wpl(n) == 2) return sizeof(double) * (size_t)split_pitch<8, 2>() * (size_t)tile.w;
     return sizeof(double) * (size_t)line_pitch(n + fast_ghost_cells(chunk_for(n))) *
            (size_t)tile.w;
 }
@@ -800,9 +1201,51 @@ static hipError_t dispatch(Dir dir, Epi epi, bool fuse, const SweepArgs &a, int 
                 : dispatch_x<CH, false>(epi, grid, block, lds, s, a, t);
 }
 
+template <int CH, int WPL, bool FUSE>
+static hipError_t dispatch_x_split(Epi epi, dim3 grid, dim3 block, size_t lds, hipStream_t s,
+                                   const SweepArgs &a, Tile t)
+{
+    const int layout = (a.in_panel ? LAYOUT_IN : 0) | (a.out_panel ? LAYOUT_OUT : 0);
+    if (layout && (epi != EPI_NONE || t.w != PANEL_W)) return hipErrorInvalidValue;
+    if (layout == LAYOUT_IN)
+        return launch(fast_x_split_kernel<CH, WPL, EPI_NONE, FUSE, LAYOUT_IN>, grid, block, lds,
+                      s, a, t);
+    if (layout == LAYOUT_OUT)
+        return launch(fast_x_split_kernel<CH, WPL, EPI_NONE, FUSE, LAYOUT_OUT>, grid, block, lds,
+                      s, a, t);
+    if (layout)
+        return launch(fast_x_split_kernel<CH, WPL, EPI_NONE, FUSE, LAYOUT_IN | LAYOUT_OUT>, grid,
+                      block, lds, s, a, t);
+    if (epi == EPI_NONE)
+        return launch(fast_x_split_kernel<CH, WPL, EPI_NONE, FUSE>, grid, block, lds, s, a, t);
+    if (epi == EPI_INIT)
+        return launch(fast_x_split_kernel<CH, WPL, EPI_INIT, FUSE>, grid, block, lds, s, a, t);
+    return launch(fast_x_split_kernel<CH, WPL, EPI_CORRECT, FUSE>, grid, block, lds, s, a, t);
+}
+
+// split lines: tile.nw groups of WPL waves
+template <int CH, int WPL>
+static hipError_t dispatch_split(Dir dir, Epi epi, bool fuse, const SweepArgs &a, int nf,
+                                 Tile t, hipStream_t s)
+{
+    dim3 grid(a.nl / t.w, nf), block(64 * t.nw * WPL);
+    size_t lds = sweep_lds_bytes_fast(a.n, t);
+    if (dir == DIR_Y) {
+        if (a.in_panel != a.out_panel || (a.in_panel && a.n % PANEL_W))
+            return hipErrorInvalidValue;
+        if (a.in_panel)
+            return launch(fast_y_split_kernel<CH, WPL, LAYOUT_IN | LAYOUT_OUT>, grid, block, lds,
+                          s, a, t);
+        return launch(fast_y_split_kernel<CH, WPL>, grid, block, lds, s, a, t);
+    }
+    return fuse ? dispatch_x_split<CH, WPL, true>(epi, grid, block, lds, s, a, t)
+                : dispatch_x_split<CH, WPL, false>(epi, grid, block, lds, s, a, t);
+}
+
 hipError_t launch_sweep_fast(Dir dir, Epi epi, const SweepArgs &a, int nfields,
                              Tile tile, hipStream_t s, bool fuse)
 {
+    if (fast_wpl(a.n) == 2) return dispatch_split<8, 2>(dir, epi, fuse, a, nfields, tile, s);
     switch (chunk_for(a.n)) {
     case 1: return dispatch<1>(dir, epi, fuse, a, nfields, tile, s);
     case 2: return dispatch<2>(dir, epi, fuse, a, nfields, tile, s);
@@ -816,7 +1259,8 @@ hipError_t launch_sweep_fast(Dir dir, Epi epi, const SweepArgs &a, int nfields,
 
 template <typename K>
 static hipError_t launch_r(K k, int nblocks, Tile t, size_t lds, hipStream_t s,
-                           const RaggedItem *items, const int *prefix, int nitems)
+                           const RaggedItem *items, const int *prefix, int nitems,
+                           int wpl = 1)
 {
     static bool attr = false;
     if (!attr) {
@@ -825,9 +1269,23 @@ static hipError_t launch_r(K k, int nblocks, Tile t, size_t lds, hipStream_t s,
         if (e != hipSuccess) return e;
         attr = true;
     }
-    hipLaunchKernelGGL(k, dim3(nblocks), dim3(64 * t.nw), lds, s, items, prefix,
+    hipLaunchKernelGGL(k, dim3(nblocks), dim3(64 * t.nw * wpl), lds, s, items, prefix,
                        nitems, t.w, t.nw);
     return hipGetLastError();
+}
+
+template <int CH, int WPL, bool FUSE>
+static hipError_t dispatch_rx_split(Epi epi, const RaggedItem *items, const int *prefix,
+                                    int nitems, int nblocks, Tile t, size_t lds, hipStream_t s)
+{
+    if (epi == EPI_NONE)
+        return launch_r(ragged_fast_x_split<CH, WPL, EPI_NONE, FUSE>, nblocks, t, lds, s, items,
+                        prefix, nitems, WPL);
+    if (epi == EPI_INIT)
+        return launch_r(ragged_fast_x_split<CH, WPL, EPI_INIT, FUSE>, nblocks, t, lds, s, items,
+                        prefix, nitems, WPL);
+    return launch_r(ragged_fast_x_split<CH, WPL, EPI_CORRECT, FUSE>, nblocks, t, lds, s, items,
+                    prefix, nitems, WPL);
 }
 
 template <int CH, bool FUSE>
@@ -855,11 +1313,22 @@ static hipError_t dispatch_r(Dir dir, Epi epi, bool fuse, const RaggedItem *item
                 : dispatch_rx<CH, false>(epi, items, prefix, nitems, nblocks, t, lds, s);
 }
 
-hipError_t launch_ragged_fast(Dir dir, Epi epi, int ch, bool fuse,
+hipError_t launch_ragged_fast(Dir dir, Epi epi, int ch, int wpl, bool fuse,
                               const RaggedItem *items, const int *prefix, int nitems,
                               int nblocks, Tile t, size_t lds, hipStream_t s)
 {
     if (nblocks <= 0 || nitems <= 0) return hipSuccess;
+    if (wpl == 2) {
+        // split lines (1024 cells, 8 per lane)
+        if (dir == DIR_Y)
+            return launch_r(ragged_fast_y_split<8, 2>, nblocks, t, lds, s, items, prefix,
+                            nitems, 2);
+        return fuse ? dispatch_rx_split<8, 2, true>(epi, items, prefix, nitems, nblocks, t, lds,
+                                                    s)
+                    : dispatch_rx_split<8, 2, false>(epi, items, prefix, nitems, nblocks, t,
+                                                     lds, s);
+    }
+    if (wpl != 1) return hipErrorInvalidValue;
     switch (ch) {
     case 1: return dispatch_r<1>(dir, epi, fuse, items, prefix, nitems, nblocks, t, lds, s);
     case 2: return dispatch_r<2>(dir, epi, fuse, items, prefix, nitems, nblocks, t, lds, s);

@@ -6,6 +6,7 @@
 #pragma once
 
 #include "ccfv_feet.h"
+#include "ccfv_internal.h"
 
 namespace ccfv {
 
@@ -18,12 +19,14 @@ struct TabScalars {
 
 // Thomas factors of the constant tridiagonal matrix of one sweep direction
 // and step size (T2S4_Parareal.cu:550-561, :583-594, :666-673); theta is 1/4
-// for the x half-steps and 1/2 for the y step.  tab (4 n + 128 doubles):
+// for the x half-steps and 1/2 for the y step.  tab (tab_doubles(n)):
 //   [mneg | den / 12 | pneg | z | mf (64) | mb (64)]
 // mneg = -a den, den the inverse pivots (with the compact operator's 1/12),
 // pneg = -p, z = T^{-1} u of the Sherman-Morrison split when periodic (else
 // 0), mf / mb the products of mneg / pneg over each lane's chunk of ch cells
-// (1 on idle lanes).  work: 2 n doubles.
+// (1 on idle lanes); for split lines (fast_wpl(n) > 1) then the products
+// over each of the 64 wpl lanes of 8 cells (tab_doubles(n) in all).  work:
+// 2 n doubles.
 __host__ __device__ inline TabScalars thomas_fill(int n, double d, double theta, double nu,
                                                   double h, bool per, int ch, double *tab,
                                                   double *work)
@@ -81,6 +84,15 @@ __host__ __device__ inline TabScalars thomas_fill(int n, double d, double theta,
         }
         tab[4 * n + l] = f;
         tab[4 * n + 64 + l] = b;
+    }
+    // split lines: the same over the 64 wpl lanes of 8 cells
+    const int ls = fast_wpl(n) > 1 ? 64 * fast_wpl(n) : 0;
+    for (int l = 0; l < ls; l++) {
+        double f = 1.0, b = 1.0;
+        for (int k = 0; k < 8; k++) f = mneg[l * 8 + k] * f;
+        for (int k = 7; k >= 0; k--) b = pneg[l * 8 + k] * b;
+        tab[4 * n + 128 + l] = f;
+        tab[4 * n + 128 + ls + l] = b;
     }
     return T;
 }
