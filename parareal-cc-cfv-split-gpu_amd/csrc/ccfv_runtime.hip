@@ -1140,7 +1140,7 @@ int ccfv_time_sweep(ccfv_ctx *c, int32_t dir, int32_t fused, int32_t nf, double 
         return r;
     };
     if (e == hipSuccess && graph) {
-        // one eager launch first: lazy per-kernel set-up stays out of the capture
+        // one eager launch first (a warm start for the replays)
         e = fused ? run_sweep_fused(c, post, EPI_NONE, a, nf, t)
                   : run_sweep(c, d, post, EPI_NONE, a, nf, t);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

@@ -290,13 +290,6 @@ template <typename K>
 static hipError_t launch(K k, dim3 grid, dim3 block, size_t lds, hipStream_t s,
                          const SweepArgs &a, Tile t)
 {
-    static bool attr = false;  // one attribute call per instantiation
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute(
-            (const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
     hipLaunchKernelGGL(k, grid, block, lds, s, a, t.w, t.nw);
     return hipGetLastError();
 }
@@ -352,13 +345,6 @@ template <typename K>
 static hipError_t launch_r(K k, int nblocks, Tile t, size_t lds, hipStream_t s,
                            const RaggedItem *items, const int *prefix, int nitems)
 {
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute(
-            (const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
     hipLaunchKernelGGL(k, dim3(nblocks), dim3(64 * t.nw), lds, s, items, prefix,
                        nitems, t.w, t.nw);
     return hipGetLastError();
