@@ -1493,10 +1493,12 @@ int parareal_pipelined(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R,
     hipStream_t sC = c->stream, sF = c->stream2;
     int rc;
 
-    auto enqueue_fine = [&](int k) -> int {
+    // groups [gb, ge) of the fine propagators of iteration k
+    auto enqueue_fine = [&](int k, int gb = 0, int ge = -1) -> int {
         const int par = k & 1;
+        if (ge < 0) ge = G;
         c->cur = sF;
-        for (int g = 0; g < G; g++) {
+        for (int g = gb; g < ge; g++) {
             const int lo = std::max(g0(g), k);
             if (lo >= g1(g)) continue;
             HIPCHK(hipStreamWaitEvent(sF, ev(par, 0, g), 0));
@@ -1513,9 +1515,10 @@ int parareal_pipelined(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R,
         c->cur = sC;
         return CCFV_OK;
     };
-    auto fine_count = [&](int k) {
+    auto fine_count = [&](int k, int gb = 0, int ge = -1) {
         double s = 0;
-        for (int g = 0; g < G; g++) {
+        if (ge < 0) ge = G;
+        for (int g = gb; g < ge; g++) {
             const int lo = std::max(g0(g), k);
             if (lo < g1(g)) s += (double)(g1(g) - lo) * J;
         }
@@ -1616,6 +1619,14 @@ int parareal_pipelined(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R,
     R.coarse_steps_run = ns;
     if (pl.kmax > 0) R.fine_steps_run += fine_count(0);
 
+    // With a tolerance (and no graphs) the last group of iteration k+1's
+    // fine propagators is enqueued only once the host has read iteration
+    // k's increment and found it above tol: that group can start only when
+    // the chain of k has passed it (its end) anyway, and a converged solve
+    // no longer runs it (half of an iteration's fine work at 64 slices of
+    // 32-slice groups).  The other groups start behind the chain as before.
+    const bool defer = pl.tol > 0 && !graphs;
+    const int gq = defer ? G - 1 : G;
     int k;
     bool pending = false;  // increments still to be read (tol <= 0)
     for (k = 0; k < pl.kmax; k++) {
@@ -1624,12 +1635,12 @@ int parareal_pipelined(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R,
         TraceRange tr(label);
         rc = phase(k, [&]() -> int {
             int r2 = enqueue_chain(k);
-            if (r2 == CCFV_OK && k + 1 < pl.kmax) r2 = enqueue_fine(k + 1);
+            if (r2 == CCFV_OK && k + 1 < pl.kmax) r2 = enqueue_fine(k + 1, 0, gq);
             return r2;
         });
         if (rc) return rc;
         R.coarse_steps_run += ns - k - 1;
-        if (k + 1 < pl.kmax) R.fine_steps_run += fine_count(k + 1);
+        if (k + 1 < pl.kmax) R.fine_steps_run += fine_count(k + 1, 0, gq);
         if (pl.tol > 0) {
             HIPCHK(hipStreamSynchronize(sC));
             double inc = increment_of(c, ns, k, slot_u2);
@@ -1643,6 +1654,11 @@ int parareal_pipelined(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R,
                 converged = true;
                 k++;
                 break;
+            }
+            if (defer && k + 1 < pl.kmax) {
+                TraceRange td("deferred fine group");
+                if ((rc = enqueue_fine(k + 1, gq, G))) return rc;
+                R.fine_steps_run += fine_count(k + 1, gq, G);
             }
         } else {
             pending = true;
@@ -1704,10 +1720,12 @@ int parareal_overlapped(ccfv_ctx *c, const ParaPlan &pl, ccfv_parareal_report &R
     double *res[2] = {c->res2, c->res2 + N};
     int rc;
 
-    auto enqueue_fine = [&](int k) -> int {
+    // groups [gb, ge) of the fine propagators of iteration k
+    auto enqueue_fine = [&](int k, int gb = 0, int ge = -1) -> int {
         const int par = k & 1;
+        if (ge < 0) ge = G;
         c->cur = sF;
-        for (int g = 0; g < G; g++) {
+        for (int g = gb; g < ge; g++) {
             const int lo = std::max(g0(g), k);
             if (lo >= g1(g)) continue;
             HIPCHK(hipStreamWaitEvent(sF, ev(par, 0, g), 0));
