@@ -33,7 +33,9 @@ cpu_baseline: the same scheme (the CPU restatement) on every host core this
 process may use, OpenMP over lines, on a bounded number of steps of the plain
 sweep, plus a one-thread sample; the CPU model is recorded.
 
-Legs (short runs of the other workloads, as extra keys): the 2048^2 swirl
+Legs (short runs of the other workloads, as extra keys): the same 512^2
+plan on a translating flow where the reference's Parareal converges (its
+speed-up over the plain sweep at matched accuracy), the 2048^2 swirl
 (plain and Parareal with graph capture and autotuned tiles), the seeded
 ragged batch of 256 independent solves, and one Poisson rate of the serving
 stream.  --workload ragged|serve|swirl run the full versions alone.
@@ -97,9 +99,10 @@ def parse():
                          "iteration (--workload swirl: 64 plain steps and one "
                          "Parareal iteration, one stream, no graphs)")
     ap.add_argument("--workload", default="rotation",
-                    choices=("rotation", "ragged", "serve", "swirl", "copy"),
+                    choices=("rotation", "translation", "ragged", "serve", "swirl", "copy"),
                     help="rotation: the headline (default, with short legs of the "
-                         "others); ragged: 256 independent solves in one batch; "
+                         "others); translation: the same plan on a translating flow, "
+                         "where Parareal converges; ragged: 256 independent solves in one batch; "
                          "serve: Poisson request streams through the serving "
                          "engine; swirl: 2048^2 deformational swirl, 128 slices, "
                          "coarse step 32 fine steps, each Parareal phase one "
@@ -644,6 +647,82 @@ def swirl(a, reps=None):
     }
 
 
+def smooth(a, reps=None):
+    """Where the reference's Parareal converges: the headline's grid, time
+    interval, fine steps, slices and tolerance (512^2, T=1 in 1024 fine
+    steps, 64 slices of 16, kmax 8, tol 1e-10), but a translating flow
+    (u, v) = (0.93, 0.47), nu = 1e-3, and a Gaussian of standard deviation
+    0.1 at (0.5, 0.5).  Plain sweep (replayed from a hipGraph) against
+    batched Parareal, each against the serial fine solution."""
+    import ccfv
+    from ccfv import types as T
+    p, ic = T.constant_gaussians(a.n, a.n, 0.93, 0.47, 1e-3, [(0.5, 0.5, 0.1, 1.0)])
+    ctx = ccfv.Context(p)
+    ctx.fill_exact(ic, 0.0)
+    u0 = ctx.get_state()
+    reps = reps or a.steps
+
+    def reset():
+        ctx.set_state(u0)
+
+    plain = lambda: ctx.plain_solve(0.0, 1.0, a.nt)  # noqa: E731
+    if not a.no_graphs:
+        ctx.set_option(T.OPT_PLAIN_GRAPHS, 1)
+    timed(ctx, reset, plain, 3)
+    plain_ms = timed(ctx, reset, plain, reps)
+    ctx.set_option(T.OPT_PLAIN_GRAPHS, 0)
+    reset()
+    plain()
+    uref = ctx.get_state()
+    box = {}
+
+    def para():
+        box["r"] = ctx.parareal_solve(0.0, 1.0, a.nt, a.slices, a.kmax, a.tol)
+
+    timed(ctx, reset, para, 3)
+    para_ms = timed(ctx, reset, para, reps)
+    rep = box["r"]
+    rel = rel_l2(ctx.get_state(), uref)
+    # the same solve with fewer, longer slices
+    by_slices = {}
+    for ns in (32, 16):
+        if a.nt % ns:
+            continue
+        r = {}
+        f = lambda: r.update(rep=ctx.parareal_solve(0.0, 1.0, a.nt, ns, a.kmax, a.tol))  # noqa: E731
+        timed(ctx, reset, f, 2)
+        ms = timed(ctx, reset, f, reps)
+        e = rel_l2(ctx.get_state(), uref)
+        by_slices[str(ns)] = {"p50_ms": pct(ms, 50), "speedup_vs_plain": pct(plain_ms, 50) / pct(ms, 50),
+                              "iterations": r["rep"].iterations,
+                              "accepted": bool(r["rep"].converged) and e <= a.tol,
+                              "rel_l2_vs_serial_fine": e}
+    ctx.close()
+    cells = a.n * a.n
+    pa, pp = pct(para_ms, 50), pct(plain_ms, 50)
+    accepted = bool(rep.converged) and rel <= a.tol
+    return {
+        "metric": "FP64 cell-updates/s, batched Parareal vs plain sweep where Parareal "
+                  "converges",
+        "value": cells * a.nt / (min(pa, pp) * 1e-3) if accepted else cells * a.nt / (pp * 1e-3),
+        "unit": "cell-updates/s", "n_gpus": 1, "steps": reps, "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic (Gaussian cell averages)",
+        "config": {"workload": f"translation-{a.n}-parareal", "grid": [a.n, a.n], "T": 1.0,
+                   "fine_steps": a.nt, "slices": a.slices, "J": a.nt // a.slices,
+                   "kmax": a.kmax, "tol": a.tol, "velocity": [0.93, 0.47], "nu": 1e-3,
+                   "bc": "periodic"},
+        "plain_graph_p50_ms": pp, "plain_graph_p99_ms": pct(plain_ms, 99),
+        "parareal_p50_ms": pa, "parareal_p99_ms": pct(para_ms, 99),
+        "speedup_vs_plain": pp / pa,
+        "parareal_iterations": rep.iterations, "parareal_converged": bool(rep.converged),
+        "parareal_accepted": accepted,
+        "parareal_increments": [float(x) for x in rep.increments],
+        "rel_l2_vs_serial_fine": rel,
+        "fine_field_steps": rep.fine_steps_run, "coarse_field_steps": rep.coarse_steps_run,
+        "parareal_by_slices": by_slices,
+    }
+
+
 def probe(a):
     """Short run for counter collection (rocprofv3 --pmc): one calibration
     copy between two 2048^2 fields that are not cache resident, 64 plain
@@ -682,6 +761,9 @@ def main():
     if a.workload == "swirl":
         print(json.dumps(swirl(a)))
         return
+    if a.workload == "translation":
+        print(json.dumps(smooth(a)))
+        return
     if a.workload == "copy":
         print(json.dumps(copy_rate(a)))
         return
@@ -692,6 +774,9 @@ def main():
     line = rotation(a)
     if not a.no_legs:
         legs = {}
+        t = time.perf_counter()
+        legs["translation-512"] = smooth(a, reps=min(a.steps, 10))
+        log(f"leg translation done ({time.perf_counter() - t:.1f} s)")
         t = time.perf_counter()
         legs["swirl-2048"] = swirl(a, reps=1)
         log(f"leg swirl done ({time.perf_counter() - t:.1f} s)")
