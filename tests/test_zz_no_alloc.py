@@ -2,7 +2,7 @@
 ccfv_plain_solve, through the C ABI: bitwise the same field for every sweep
 path (uniform-offset periodic rotation, the general sweep of the swirl, the
 closed reference box with its quirks, a non-square constant-velocity
-problem), on two calls in a row with the second on a smaller grid; the
+problem, 1024-cell split lines), on two calls in a row with the second on a smaller grid; the
 largest grid (2048 x 2048) fits its static scratch; bad arguments are
 refused before any launch; crossed feet are reported as by
 ccfv_plain_solve, and the next call starts from a cleared flag.  The
@@ -26,6 +26,10 @@ def cases():
         ("swirl-128", T.swirl_problem(128, T=1.0), T.cosine_bell_ic(), 0.0, 0.5, 64),
         ("reference-32", T.reference_problem(32, 0.01), T.reference_ic(), 0.0, 0.3, 24),
         ("gauss-96x40", g, gic, 0.1, 0.4, 30),
+        # x lines of 1024 cells: split lines, whose chunk products the
+        # device builds in the static scratch with the rest of the table
+        ("gauss-1024x64", *T.constant_gaussians(1024, 64, 0.7, -0.4, 2e-3,
+                                                [(0.3, 0.6, 0.08, 1.0)]), 0.0, 0.05, 20),
     ]
 
 
@@ -44,7 +48,7 @@ def noalloc(gpu, p, u0, t0, t1, nt):
         return c.get_state()
 
 
-@pytest.mark.parametrize("first,second", [(0, 2), (0, 1), (1, 2), (3, 2)])
+@pytest.mark.parametrize("first,second", [(0, 2), (0, 1), (1, 2), (3, 2), (4, 0)])
 def test_noalloc_equals_plain_solve_twice(gpu, first, second):
     """Two calls in a row on one GPU (its static scratch keeps the first
     call's tables), the second on a smaller grid."""
