@@ -508,14 +508,26 @@ __device__ inline void block_reduce2(double s0, double s1, double *partial, int 
 }
 
 
-// item of block b in a ragged launch: prefix[i] <= b < prefix[i+1]
+// item of block b in a ragged launch: the last i with prefix[i] <= b, so
+// prefix[i] <= b < prefix[i+1] (prefix[0] = 0, non-decreasing).  Every
+// wave searches on its own, 64 entries per round: the lanes test evenly
+// spaced candidates of [lo, hi] at once and the ballot keeps the last one
+// that is <= b, so 64 items take one dependent load and 4096 two (a binary
+// search takes log2(nitems)).  Call with all 64 lanes of the wave active.
 __device__ __forceinline__ int ragged_item(const int *prefix, int nitems, int b)
 {
-    int lo = 0, hi = nitems - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (prefix[mid] <= b) lo = mid;
-        else hi = mid - 1;
+    const int lane = threadIdx.x & 63;
+    int lo = 0, hi = nitems - 1;  // prefix[lo] <= b; the answer is in [lo, hi]
+    while (hi > lo) {
+        const int step = (hi - lo + 64) >> 6;  // ceil((hi - lo + 1) / 64)
+        const int k = lo + lane * step;
+        const bool ok = k <= hi && prefix[k] <= b;
+        // the candidates are ascending and prefix is non-decreasing: the lanes
+        // that pass are 0 .. last (lane 0 passes: k = lo)
+        const unsigned long long m = __ballot(ok);
+        const int last = 63 - __clzll(m);
+        lo += last * step;
+        hi = min(hi, lo + step - 1);
     }
     return lo;
 }
