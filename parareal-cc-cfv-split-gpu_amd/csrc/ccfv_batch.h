@@ -72,12 +72,6 @@ struct Phase {
 // (with the lanes' fork and join) and from then on replayed with one graph
 // launch; the kernels read the new descriptors from the arena, so only the
 // launch list has to repeat.
-// Without graphs the upload goes to a stream of its own (`up_stream`), so it
-// overlaps the lanes' work of the phase before, and each lane waits only
-// for the upload of its phase; a phase run with join = false also leaves
-// the lanes running on (a lane's next phase follows its own launches), and
-// the next joined phase — or join() — brings every lane with unjoined work
-// back into `stream`.
 struct PhaseRunner {
     static constexpr int LANES = 4;
     hipStream_t stream = nullptr;
@@ -92,14 +86,6 @@ struct PhaseRunner {
     char *dbuf[2] = {nullptr, nullptr};
     size_t cap[2] = {0, 0};
     hipEvent_t done[2] = {nullptr, nullptr};
-    // eager phases: the upload stream, the end of each arena's upload, the
-    // end of each lane's launches of the phase that read the arena, and the
-    // lanes whose launches `stream` has not yet waited for
-    hipStream_t up_stream = nullptr;
-    hipEvent_t up_ev[2] = {nullptr, nullptr};
-    hipEvent_t lane_done[2][LANES] = {};
-    bool lane_used[2][LANES] = {};
-    bool pending[LANES] = {};
     int slot = 0;
     bool graphs = false;
     std::map<std::string, hipGraphExec_t> cache;
@@ -108,14 +94,8 @@ struct PhaseRunner {
     int init();
     void release();
     void clear_cache();
-    // join = false: `stream` does not wait for the other lanes at the end of
-    // the phase (the caller runs a joined phase or join() before it reads
-    // results on `stream`); ignored with graphs on (every phase joins)
-    int run(Phase &ph, bool join = true);
+    int run(Phase &ph);
     int issue(const Phase &ph, const char *d);
-    int issue_eager(const Phase &ph, const char *d, int s, bool join);
-    // `stream` waits for every lane's launches so far
-    int join();
 };
 
 // A run of chained Strang steps of one problem (as run_steps), emitted into

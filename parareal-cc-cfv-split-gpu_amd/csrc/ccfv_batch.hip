@@ -143,18 +143,13 @@ int ccfv_batch_plain(ccfv_batch *b, const double *t0, const double *t1,
                                                    0.0, h};
                                }});
     }
-    // phases of 64 rounds keep the descriptor upload small; the lanes run on
-    // across phases (each problem's steps stay in its lane's order) and join
-    // `stream` after the last
+    // phases of 64 rounds keep the descriptor upload small
     Phase ph;
     const int rounds = run_rounds(runs);
     for (int r0 = 0; r0 < rounds; r0 += 64) {
         ph.clear();
-        const int r1 = std::min(rounds, r0 + 64);
-        if ((rc = emit_runs(ph, runs, r0, r1)) || (rc = b->run.run(ph, r1 == rounds))) {
-            (void)b->run.join();
-            return rc;
-        }
+        if ((rc = emit_runs(ph, runs, r0, std::min(rounds, r0 + 64)))) return rc;
+        if ((rc = b->run.run(ph))) return rc;
     }
     HIPCHK(hipStreamSynchronize(b->run.stream));
     for (ccfv_ctx *c : b->ctx)

@@ -191,12 +191,6 @@ int PhaseRunner::init()
         HIPCHK(hipEventCreateWithFlags(&join_ev[l], hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
-    HIPCHK(hipStreamCreateWithFlags(&up_stream, hipStreamNonBlocking));
-    for (int s = 0; s < 2; s++) {
-        HIPCHK(hipEventCreateWithFlags(&up_ev[s], hipEventDisableTiming));
-        for (int l = 0; l < LANES; l++)
-            HIPCHK(hipEventCreateWithFlags(&lane_done[s][l], hipEventDisableTiming));
-    }
     return CCFV_OK;
 }
 
@@ -224,25 +218,15 @@ void PhaseRunner::release()
 {
     for (int l = 0; l < LANES; l++)
         if (lanes[l]) (void)hipStreamSynchronize(lanes[l]);
-    if (up_stream) (void)hipStreamSynchronize(up_stream);
     clear_cache();
     for (int s = 0; s < 2; s++) {
         if (hbuf[s]) (void)hipHostFree(hbuf[s]);
         if (dbuf[s]) (void)hipFree(dbuf[s]);
         if (done[s]) (void)hipEventDestroy(done[s]);
-        if (up_ev[s]) (void)hipEventDestroy(up_ev[s]);
         hbuf[s] = dbuf[s] = nullptr;
-        done[s] = up_ev[s] = nullptr;
+        done[s] = nullptr;
         cap[s] = 0;
-        for (int l = 0; l < LANES; l++) {
-            if (lane_done[s][l]) (void)hipEventDestroy(lane_done[s][l]);
-            lane_done[s][l] = nullptr;
-            lane_used[s][l] = false;
-        }
     }
-    for (int l = 0; l < LANES; l++) pending[l] = false;
-    if (up_stream) (void)hipStreamDestroy(up_stream);
-    up_stream = nullptr;
     for (int l = 1; l < LANES; l++) {
         if (lanes[l]) (void)hipStreamDestroy(lanes[l]);
         if (join_ev[l]) (void)hipEventDestroy(join_ev[l]);
@@ -255,21 +239,14 @@ void PhaseRunner::release()
     stream = lanes[0] = nullptr;
 }
 
-int PhaseRunner::run(Phase &ph, bool join_after)
+int PhaseRunner::run(Phase &ph)
 {
-    if (ph.launches.empty()) return join_after ? join() : CCFV_OK;
+    if (ph.launches.empty()) return CCFV_OK;
     const size_t bytes = ph.host.size();
     const int s = slot;
     slot ^= 1;
-    // this arena was last uploaded for, and read by, the phase before the
-    // previous one
+    // this arena was last read by the phase before the previous one
     HIPCHK(hipEventSynchronize(done[s]));
-    HIPCHK(hipEventSynchronize(up_ev[s]));
-    for (int l = 0; l < LANES; l++)
-        if (lane_used[s][l]) {
-            HIPCHK(hipEventSynchronize(lane_done[s][l]));
-            lane_used[s][l] = false;
-        }
     if (bytes > cap[s]) {
         // captured phases hold addresses in the arena that goes away
         HIPCHK(hipStreamSynchronize(stream));
@@ -285,19 +262,9 @@ int PhaseRunner::run(Phase &ph, bool join_after)
         cap[s] = c;
     }
     memcpy(hbuf[s], ph.host.data(), bytes);
-    const char *d = dbuf[s];
-    if (!graphs) {
-        HIPCHK(hipMemcpyAsync(dbuf[s], hbuf[s], bytes, hipMemcpyHostToDevice, up_stream));
-        HIPCHK(hipEventRecord(up_ev[s], up_stream));
-        int rc = issue_eager(ph, d, s, join_after);
-        if (rc) return rc;
-        eager_launches++;
-        return CCFV_OK;
-    }
-    // graphs: every phase joins (lanes with unjoined eager work first)
-    if (int rc = join()) return rc;
     HIPCHK(hipMemcpyAsync(dbuf[s], hbuf[s], bytes, hipMemcpyHostToDevice, stream));
-    {
+    const char *d = dbuf[s];
+    if (graphs) {
         // the launch list and the arena it reads
         std::string key((const char *)&d, sizeof(d));
         for (const Launch &L : ph.launches) {
@@ -334,31 +301,6 @@ int PhaseRunner::run(Phase &ph, bool join_after)
     return CCFV_OK;
 }
 
-// one launch of a phase on stream ls; d: the phase's descriptors on the device
-static hipError_t launch_one(const Launch &L, const char *d, hipStream_t ls)
-{
-    const int *pf = (const int *)(d + L.prefix);
-    switch (L.kind) {
-    case Launch::SWEEP: {
-        const RaggedItem *it = (const RaggedItem *)(d + L.items);
-        return L.fast ? launch_ragged_fast(L.dir, L.epi, L.ch, L.wpl, L.fuse, it, pf, L.nitems,
-                                           L.nblocks, L.tile, L.lds, ls)
-                      : launch_ragged_generic(L.dir, L.bc, L.post, L.epi, L.ch, L.fuse, it, pf,
-                                              L.nitems, L.nblocks, L.tile, L.lds, ls);
-    }
-    case Launch::CORR:
-        return launch_ragged_correction((const RaggedCorr *)(d + L.items), pf, L.nitems,
-                                        L.nblocks, ls);
-    case Launch::REDUCE:
-        return launch_ragged_reduce((const RaggedReduce *)(d + L.items), pf, L.nitems,
-                                    L.nblocks, ls);
-    case Launch::COPY:
-        return launch_ragged_copy((const RaggedCopy *)(d + L.items), pf, L.nitems, L.nblocks,
-                                  ls);
-    }
-    return hipErrorInvalidValue;
-}
-
 int PhaseRunner::issue(const Phase &ph, const char *d)
 {
     // fork: the lanes in use start after what `stream` holds (the upload)
@@ -369,7 +311,32 @@ int PhaseRunner::issue(const Phase &ph, const char *d)
         if (used[l]) HIPCHK(hipStreamWaitEvent(lanes[l], fork_ev, 0));
     int rc = CCFV_OK;
     for (const Launch &L : ph.launches) {
-        const hipError_t e = launch_one(L, d, lanes[L.lane]);
+        hipError_t e = hipSuccess;
+        const hipStream_t ls = lanes[L.lane];
+        const int *pf = (const int *)(d + L.prefix);
+        switch (L.kind) {
+        case Launch::SWEEP: {
+            const RaggedItem *it = (const RaggedItem *)(d + L.items);
+            e = L.fast ? launch_ragged_fast(L.dir, L.epi, L.ch, L.wpl, L.fuse, it, pf,
+                                            L.nitems, L.nblocks, L.tile, L.lds, ls)
+                       : launch_ragged_generic(L.dir, L.bc, L.post, L.epi, L.ch, L.fuse, it,
+                                               pf, L.nitems, L.nblocks, L.tile, L.lds,
+                                               ls);
+            break;
+        }
+        case Launch::CORR:
+            e = launch_ragged_correction((const RaggedCorr *)(d + L.items), pf, L.nitems,
+                                         L.nblocks, ls);
+            break;
+        case Launch::REDUCE:
+            e = launch_ragged_reduce((const RaggedReduce *)(d + L.items), pf, L.nitems,
+                                     L.nblocks, ls);
+            break;
+        case Launch::COPY:
+            e = launch_ragged_copy((const RaggedCopy *)(d + L.items), pf, L.nitems,
+                                   L.nblocks, ls);
+            break;
+        }
         if (e != hipSuccess) {
             rc = hip_fail(e, "ragged launch");
             break;
@@ -383,56 +350,6 @@ int PhaseRunner::issue(const Phase &ph, const char *d)
             if (e == hipSuccess) e = hipStreamWaitEvent(stream, join_ev[l], 0);
             if (e != hipSuccess && rc == CCFV_OK) rc = hip_fail(e, "phase lane join");
         }
-    return rc;
-}
-
-int PhaseRunner::join()
-{
-    for (int l = 1; l < LANES; l++)
-        if (pending[l]) {
-            HIPCHK(hipEventRecord(join_ev[l], lanes[l]));
-            HIPCHK(hipStreamWaitEvent(stream, join_ev[l], 0));
-            pending[l] = false;
-        }
-    return CCFV_OK;
-}
-
-int PhaseRunner::issue_eager(const Phase &ph, const char *d, int s, bool join_after)
-{
-    bool used[LANES] = {};
-    for (const Launch &L : ph.launches) used[L.lane] = true;
-    // a lane that `stream` has caught up with starts after what `stream`
-    // holds; a lane still running unjoined phases follows its own order.
-    // Every lane waits for the upload of this phase's descriptors.
-    bool fork = false;
-    for (int l = 1; l < LANES; l++) fork = fork || (used[l] && !pending[l]);
-    if (fork) HIPCHK(hipEventRecord(fork_ev, stream));
-    for (int l = 0; l < LANES; l++) {
-        if (!used[l]) continue;
-        if (l > 0 && !pending[l]) HIPCHK(hipStreamWaitEvent(lanes[l], fork_ev, 0));
-        HIPCHK(hipStreamWaitEvent(lanes[l], up_ev[s], 0));
-    }
-    int rc = CCFV_OK;
-    for (const Launch &L : ph.launches) {
-        const hipError_t e = launch_one(L, d, lanes[L.lane]);
-        if (e != hipSuccess) {
-            rc = hip_fail(e, "ragged launch");
-            break;
-        }
-    }
-    // the arena may be rewritten once every lane that read it is done; a
-    // failed launch joins every lane, so that `stream` covers all of them
-    for (int l = 0; l < LANES; l++)
-        if (used[l]) {
-            hipError_t e = hipEventRecord(lane_done[s][l], lanes[l]);
-            if (e != hipSuccess && rc == CCFV_OK) rc = hip_fail(e, "phase lane event");
-            if (e == hipSuccess) lane_used[s][l] = true;
-            if (l > 0) pending[l] = true;
-        }
-    if (join_after || rc != CCFV_OK) {
-        int r2 = join();
-        if (rc == CCFV_OK) rc = r2;
-    }
     return rc;
 }
 
