@@ -199,3 +199,43 @@ def test_batch_graphs_and_tuned_shapes_match_solo(gpu, tune):
                 ref, rr = solo_parareal(gpu, p, ic, 0.0, t1[i], nt[i], ns[i], 3, 0.0)
                 assert np.array_equal(para[i], ref), (rep_i, i)
                 np.testing.assert_allclose(reps[i].increments, rr.increments, rtol=1e-12)
+
+
+def many_small(count):
+    """`count` problems of one grid class (so one lane and one ragged launch
+    per stage holds all of them) with their own velocity, diffusion and
+    initial data."""
+    rng = np.random.default_rng(7)
+    out = []
+    for _ in range(count):
+        u, v = rng.uniform(-1.0, 1.0, 2)
+        nu = 10.0 ** rng.uniform(-4.0, -2.0)
+        bumps = [(float(rng.uniform()), float(rng.uniform()), float(rng.uniform(0.05, 0.1)),
+                  float(rng.uniform(0.5, 1.0)))]
+        out.append(T.constant_gaussians(32, 32, float(u), float(v), float(nu), bumps))
+    return out
+
+
+@pytest.mark.parametrize("count", [65, 150])
+def test_batch_many_items_per_launch(gpu, count):
+    """More items in one ragged launch than one round of the wave-parallel
+    item search covers (64): every block must still find its own problem.
+    Step counts differ, so the item lists shrink round by round."""
+    m = many_small(count)
+    t0 = [0.0] * count
+    t1 = [0.125] * count
+    nt = [8 + (i % 9) for i in range(count)]
+    with gpu.Batch([p for p, _ in m]) as b:
+        for i, (_, ic) in enumerate(m):
+            b[i].fill_exact(ic, t0[i])
+        b.plain(t0, t1, nt)
+        got = [b[i].get_state() for i in range(count)]
+        for i, (_, ic) in enumerate(m):
+            b[i].fill_exact(ic, t0[i])
+        reps = b.parareal(t0, t1, [16] * count, [4] * count, 2, 0.0)
+        got_p = [b[i].get_state() for i in range(count)]
+    for i, (p, ic) in enumerate(m):
+        assert np.array_equal(got[i], solo_plain(gpu, p, ic, t0[i], t1[i], nt[i])), i
+        ref, rep = solo_parareal(gpu, p, ic, 0.0, 0.125, 16, 4, 2, 0.0)
+        assert np.array_equal(got_p[i], ref), i
+        assert reps[i].iterations == rep.iterations
