@@ -44,3 +44,47 @@ def test_swirl_parareal_1024_matches_oracle(gpu, oracle):
     assert rel_l2(g, r["u"]) <= 1e-12
     for a, b in zip(rep.increments, r["increments"]):
         assert abs(a - b) <= 1e-9 * max(abs(b), 1e-30) + 1e-15
+
+
+def restrict(u):
+    """cell averages of the grid with twice the cell width (2 x 2 means)"""
+    n = u.shape[0]
+    return u.reshape(n // 2, 2, n // 2, 2).mean(axis=(1, 3))
+
+
+def swirl_to_T(gpu, n, nu):
+    """The whole swirl horizon T = 5 at the benchmark's CFL (4096 steps at
+    2048^2, proportionally fewer on coarser grids): initial and final field."""
+    p, ic = T.swirl_problem(n, T=5.0, nu=nu), T.cosine_bell_ic()
+    nt = 4096 * n // 2048
+    print(f"swirl {n}^2 nu {nu} plain {nt} steps", flush=True)
+    with gpu.Context(p) as c:
+        c.fill_exact(ic, 0.0)
+        u0 = c.get_state()
+        c.plain_solve(0.0, 5.0, nt)
+        return u0.reshape(n, n), c.get_state().reshape(n, n)
+
+
+def test_swirl_full_horizon_2048(gpu):
+    """The 2048^2 swirl over its whole horizon (4096 steps), checked by what
+    the flow fixes at every size.  Without diffusion the swirl reverses at
+    T, so u(T) must return to u(0), with an error that falls as the grid is
+    refined (measured 6.3e-3, 1.4e-3, 3.5e-4 at 512, 1024, 2048: second
+    order, the cosine bell's second derivative jumps at its rim).  With the
+    benchmark's diffusion (nu = 1e-4) the answer is no longer u(0): the
+    solutions must converge to each other (cell averages of the finer grid,
+    restricted, against the coarser: measured 3.1e-5 for 1024 -> 512 and
+    5.5e-6 for 2048 -> 1024).  Periodic: the sum of the cell averages is
+    conserved to rounding in both."""
+    rev, sol = {}, {}
+    for n in (512, 1024, 2048):
+        u0, u = swirl_to_T(gpu, n, 0.0)
+        assert abs(u.sum() - u0.sum()) <= 1e-11 * np.abs(u0).sum()
+        rev[n] = rel_l2(u, u0)
+        u0, sol[n] = swirl_to_T(gpu, n, 1e-4)
+        assert abs(sol[n].sum() - u0.sum()) <= 1e-11 * np.abs(u0).sum()
+    assert rev[1024] <= rev[512] / 3 and rev[2048] <= rev[1024] / 3, rev
+    assert rev[2048] <= 1e-3, rev
+    d1 = rel_l2(restrict(sol[1024]), sol[512])
+    d2 = rel_l2(restrict(sol[2048]), sol[1024])
+    assert d2 <= d1 / 3 and d2 <= 2e-5, (d1, d2)
