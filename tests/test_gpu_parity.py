@@ -165,3 +165,30 @@ def test_crossed_feet_are_an_error_and_recoverable(gpu, oracle):
         c.step(0.0, 0.01)
         g = c.get_state()
     assert rel_l2(g, oracle.step(p, u0, 0.0, 0.01)) <= 1e-13
+
+
+@pytest.mark.parametrize("n", [256, 512])
+def test_reference_preset_large_matches_oracle(gpu, oracle, n):
+    """The reference program's own problem (closed box [-1,1]^2, V = (-4y,
+    4x), K = 0.01, Gaussian at (-0.35, 0), both quirks on) at the grid sizes
+    its config file is run with, where the general sweep's lanes hold 4 and 8
+    cells and stage the right-hand side in LDS: 16 serial fine steps, and a
+    Parareal solve of 4 slices x 4 steps with 2 iterations, against the
+    oracle's restatement of the reference loops."""
+    p, ic = T.reference_problem(n, 0.01), T.reference_ic()
+    u0 = oracle.exact(p, ic, 0.0)
+    t1, nt = 16.0 / 512, 16
+    print(f"reference preset {n}^2: plain {nt} steps, Parareal 4 x 4", flush=True)
+    with gpu.Context(p) as c:
+        c.set_state(u0)
+        c.plain_solve(0.0, t1, nt)
+        g = c.get_state()
+        c.set_state(u0)
+        rep = c.parareal_solve(0.0, t1, nt, 4, 2)
+        gp = c.get_state()
+    assert rel_l2(g, oracle.plain(p, u0, 0.0, t1, nt)) <= 1e-12
+    r = oracle.parareal(p, u0, 0.0, t1, nt, 4, 2)
+    assert rep.iterations == 2
+    assert rel_l2(gp, r["u"]) <= 1e-12
+    for a, b in zip(rep.increments, r["increments"]):
+        assert abs(a - b) <= 1e-9 * abs(b)
