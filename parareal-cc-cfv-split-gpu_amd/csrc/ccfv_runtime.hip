@@ -727,7 +727,10 @@ int ccfv_create(const ccfv_problem *p, ccfv_ctx **out)
         return fail(CCFV_EDEVICE, "no HIP device available");
     ccfv_ctx *c = new ccfv_ctx();
     c->P = *p;
-    HIPCHK(hipGetDevice(&c->device));
+    if (hipError_t e0 = hipGetDevice(&c->device); e0 != hipSuccess) {
+        delete c;
+        return hip_fail(e0, "hipGetDevice");
+    }
     c->ncell = (size_t)p->nx * p->ny;
     const bool post = (p->flags & CCFV_FLAG_STEP3_POSTREMAP) != 0;
     const bool src = p->src_kind != CCFV_SRC_NONE;
