@@ -99,9 +99,12 @@ def parse():
                          "iteration (--workload swirl: 64 plain steps and one "
                          "Parareal iteration, one stream, no graphs)")
     ap.add_argument("--workload", default="rotation",
-                    choices=("rotation", "translation", "ragged", "serve", "swirl", "copy"),
+                    choices=("rotation", "translation", "ragged", "serve", "swirl", "copy",
+                             "accuracy"),
                     help="rotation: the headline (default, with short legs of the "
-                         "others); translation: the same plan on a translating flow, "
+                         "others); accuracy: the plain method on the periodic sin*sin "
+                         "problem, errors and orders on 32^2..256^2; "
+                         "translation: the same plan on a translating flow, "
                          "where Parareal converges; ragged: 256 independent solves in one batch; "
                          "serve: Poisson request streams through the serving "
                          "engine; swirl: 2048^2 deformational swirl, 128 slices, "
@@ -647,6 +650,58 @@ def swirl(a, reps=None):
     }
 
 
+def accuracy(a, reps=None):
+    """The plain method on its accuracy workload: periodic [0,1]^2, velocity
+    (1, 0.5), nu = 0.01, u0 = sin(2 pi x) sin(2 pi y), T = 1 at CFL 0.8
+    (40 / 80 / 160 / 320 steps on 32^2 .. 256^2), against the exact
+    decaying translated solution's cell averages (the library's own closed
+    form, ccfv_fill_exact): relative L2 and RMS errors and the observed
+    orders; the same at a fixed small step (2560 steps), where the spatial
+    order shows; and the 128^2 solve's rate (p50 of `reps` solves)."""
+    import ccfv
+    from ccfv import types as T
+    reps = reps or max(1, min(a.steps, 10))
+    rows, fixed = [], []
+    for n in (32, 64, 128, 256):
+        p, ic = T.advection_sinsin_problem(n), T.advection_sinsin_ic()
+        nt = T.advection_sinsin_steps(n)
+        with ccfv.Context(p) as c:
+            ex = c.exact(ic, 1.0)
+            ms = []
+            for r in range(reps + 1 if n == 128 else 1):
+                c.fill_exact(ic, 0.0)
+                c.sync()
+                t = time.perf_counter()
+                c.plain_solve(0.0, 1.0, nt)
+                c.sync()
+                if r >= 1 or n != 128:
+                    ms.append((time.perf_counter() - t) * 1e3)
+            u = c.get_state()
+            row = {"n": n, "steps": nt, "rel_l2": rel_l2(u, ex),
+                   "rms": float(np.sqrt(np.mean((u - ex) ** 2)))}
+            if n == 128:
+                row["p50_ms"] = pct(ms, 50)
+                row["cell_updates_per_s"] = n * n * nt / (pct(ms, 50) * 1e-3)
+            rows.append(row)
+            if n <= 128:
+                c.fill_exact(ic, 0.0)
+                c.plain_solve(0.0, 1.0, 2560)
+                fixed.append({"n": n, "steps": 2560,
+                              "rms": float(np.sqrt(np.mean((c.get_state() - ex) ** 2)))})
+    order = lambda e: [float(np.log2(e[i]["rms"] / e[i + 1]["rms"])) for i in range(len(e) - 1)]
+    r128 = rows[2]
+    return {
+        "metric": "FP64 cell-updates/s and observed order, plain method at CFL 0.8",
+        "value": r128["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": reps, "higher_is_better": True, "dtype": "f64",
+        "data": "synthetic (sin(2 pi x) sin(2 pi y) cell averages)",
+        "config": {"workload": "sinsin-128-plain", "velocity": [1.0, 0.5], "nu": 0.01,
+                   "cfl": 0.8, "T": 1.0, "bc": "periodic", "grids": [32, 64, 128, 256]},
+        "errors_vs_exact": rows, "orders_cfl_0.8": order(rows),
+        "errors_fixed_step": fixed, "orders_fixed_step": order(fixed),
+    }
+
+
 def smooth(a, reps=None):
     """Where the reference's Parareal converges: the headline's grid, time
     interval, fine steps, slices and tolerance (512^2, T=1 in 1024 fine
@@ -767,6 +822,9 @@ def main():
     if a.workload == "copy":
         print(json.dumps(copy_rate(a)))
         return
+    if a.workload == "accuracy":
+        print(json.dumps(accuracy(a)))
+        return
     if a.probe:
         print(json.dumps(probe(a)))
         return
@@ -774,6 +832,9 @@ def main():
     line = rotation(a)
     if not a.no_legs:
         legs = {}
+        t = time.perf_counter()
+        legs["sinsin-128-accuracy"] = accuracy(a, reps=min(a.steps, 10))
+        log(f"leg accuracy done ({time.perf_counter() - t:.1f} s)")
         t = time.perf_counter()
         legs["translation-512"] = smooth(a, reps=min(a.steps, 10))
         log(f"leg translation done ({time.perf_counter() - t:.1f} s)")
