@@ -167,7 +167,7 @@ def test_crossed_feet_are_an_error_and_recoverable(gpu, oracle):
     assert rel_l2(g, oracle.step(p, u0, 0.0, 0.01)) <= 1e-13
 
 
-@pytest.mark.parametrize("n", [256, 512])
+@pytest.mark.parametrize("n", [256, 512, 1024])
 def test_reference_preset_large_matches_oracle(gpu, oracle, n):
     """The reference program's own problem (closed box [-1,1]^2, V = (-4y,
     4x), K = 0.01, Gaussian at (-0.35, 0), both quirks on) at the grid sizes
@@ -192,3 +192,18 @@ def test_reference_preset_large_matches_oracle(gpu, oracle, n):
     assert rel_l2(gp, r["u"]) <= 1e-12
     for a, b in zip(rep.increments, r["increments"]):
         assert abs(a - b) <= 1e-9 * abs(b)
+
+
+def test_reference_preset_2048_matches_oracle(gpu, oracle):
+    """The reference program's problem at 2048^2: the general sweep's
+    32-cell lanes with the right-hand side and the third-sub-step line in
+    LDS (two auxiliary lines per wave), 8 serial fine steps."""
+    p, ic = T.reference_problem(2048, 0.01), T.reference_ic()
+    u0 = oracle.exact(p, ic, 0.0)
+    t1, nt = 8.0 / 2048, 8
+    print("reference preset 2048^2: plain 8 steps", flush=True)
+    with gpu.Context(p) as c:
+        c.set_state(u0)
+        c.plain_solve(0.0, t1, nt)
+        g = c.get_state()
+    assert rel_l2(g, oracle.plain(p, u0, 0.0, t1, nt)) <= 1e-12
