@@ -521,7 +521,9 @@ __device__ __forceinline__ int ragged_item(const int *prefix, int nitems, int b)
     while (hi > lo) {
         const int step = (hi - lo + 64) >> 6;  // ceil((hi - lo + 1) / 64)
         const int k = lo + lane * step;
-        const bool ok = k <= hi && prefix[k] <= b;
+        // candidates past hi fail; their address is clamped so that no lane
+        // reads past the prefix array
+        const bool ok = k <= hi && prefix[min(k, hi)] <= b;
         // the candidates are ascending and prefix is non-decreasing: the lanes
         // that pass are 0 .. last (lane 0 passes: k = lo)
         const unsigned long long m = __ballot(ok);
